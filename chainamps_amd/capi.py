@@ -1,0 +1,366 @@
+"""ctypes bindings of include/chainamps_amd.h.
+
+Mirrors the reference operator interface for the hot path:
+``contract`` (MPX_matrix.cpp:744-791), ``reorder`` (630-638),
+``Heff`` = TwoVertexComponents (DMRG_routines.cpp:779-974) with its matvec
+and lowest-eigenpair solve (replacing arpack_eigs, arpack_interface.hpp:85),
+``svd_truncate`` = MPX_matrix::SVD (MPX_matrix.cpp:157-192) and
+``tebd_gate`` = iTEBD::apply_and_decompose (TEBD_routines.cpp:233-266).
+Errors raise CamError carrying cam_last_error() (the reference prints and
+exits).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from collections import namedtuple
+from pathlib import Path
+
+import numpy as np
+
+CAM_FP64 = 0
+CAM_FP32 = 1
+CAM_MEM_HOST = 0
+CAM_MEM_DEVICE = 1
+
+_LIB = None
+_HERE = Path(__file__).resolve().parent
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("CAM_LIBRARY", _HERE / "lib" / "libchainamps_amd.so"))
+
+
+class CamError(RuntimeError):
+    pass
+
+
+class _IndexDesc(C.Structure):
+    _fields_ = [("inward", C.c_int32), ("physical", C.c_int32), ("size", C.c_int64),
+                ("charges", C.POINTER(C.c_int16))]
+
+
+IndexDesc = namedtuple("IndexDesc", "inward physical charges")
+
+_p = C.c_void_p
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+
+_SIGS = {
+    "cam_ctx_create": [C.c_int, C.POINTER(_p)],
+    "cam_ctx_destroy": [_p],
+    "cam_ctx_sync": [_p],
+    "cam_ctx_stream": [_p, C.POINTER(_p)],
+    "cam_tensor_from_csc": [_p, C.c_int, C.POINTER(C.c_int16), C.c_int, C.POINTER(_IndexDesc), C.c_int,
+                            C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.POINTER(_p)],
+    "cam_tensor_shape": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "cam_tensor_index": [_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _i64p,
+                         C.POINTER(C.c_int16)],
+    "cam_tensor_nnz": [_p, _i64p],
+    "cam_tensor_to_csc": [_p, C.c_int64, _i64p, _i64p, _dp, _i64p],
+    "cam_tensor_destroy": [_p],
+    "cam_tensor_shift_num_row_indices": [_p, C.c_int],
+    "cam_tensor_remove_dummy_indices": [_p, C.c_int, _i64p],
+    "cam_tensor_move_dummy_indices": [_p, C.c_int, _i64p],
+    "cam_tensor_rescale": [_p, C.c_double, C.c_double],
+    "cam_contract": [_p, _p, C.c_int, _p, C.c_int, C.c_int, _i64p, C.POINTER(_p)],
+    "cam_reorder": [_p, _p, C.c_int, C.c_int, _i64p, C.c_int, C.POINTER(_p)],
+    "cam_heff_create": [_p, _p, _p, _p, C.POINTER(_p), _i64p],
+    "cam_heff_destroy": [_p],
+    "cam_heff_allowed_map": [_p, _i64p],
+    "cam_heff_flops": [_p, _dp],
+    "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
+    "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
+    "cam_heff_to_blocked": [_p, _p, _p],
+    "cam_heff_from_blocked": [_p, _p, _p],
+    "cam_heff_lowest": [_p, _p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _p, _dp],
+    "cam_svd_truncate": [_p, _p, C.c_int64, C.c_double, C.POINTER(_p), C.POINTER(_p), _dp, C.c_int64,
+                         _i64p, _dp],
+    "cam_tebd_gate": [_p, _p, _p, _p, _dp, C.c_int64, C.c_int64, C.c_double, C.POINTER(_p),
+                      C.POINTER(_p), _dp, C.c_int64, _i64p, _dp],
+    "cam_heff_vjp": [_p, _p, _p, _p, C.c_int64],
+}
+
+
+def load_library():
+    """Load the HIP library; raises CamError when it is missing (there is
+    no CPU fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not path.exists():
+        raise CamError(f"chainamps_amd native library not built: {path} "
+                       "(run `python -m chainamps_amd.build`)")
+    lib = C.CDLL(str(path))
+    for name, args in _SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.cam_last_error.restype = C.c_char_p
+    lib.cam_last_error.argtypes = []
+    lib.cam_version.restype = C.c_char_p
+    lib.cam_version.argtypes = []
+    _LIB = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise CamError(_LIB.cam_last_error().decode())
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Context:
+    """cam_ctx: one device, one stream."""
+
+    def __init__(self, device=0):
+        lib = load_library()
+        h = _p()
+        _check(lib.cam_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        _check(_LIB.cam_ctx_sync(self.h))
+
+    def stream(self):
+        s = _p()
+        _check(_LIB.cam_ctx_stream(self.h, C.byref(s)))
+        return s.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tensor:
+    """Device QN-blocked tensor (cam_tensor)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    @staticmethod
+    def from_csc(ctx, rules, indices, numrows, rows, cols, p, i, x):
+        """indices: list of IndexDesc(inward, physical, charges[size, nrules]);
+        x: complex128 array."""
+        rules = np.ascontiguousarray(rules, dtype=np.int16)
+        nr = len(rules)
+        descs = (_IndexDesc * max(len(indices), 1))()
+        keep = []
+        for k, ix in enumerate(indices):
+            ch = np.ascontiguousarray(np.asarray(ix.charges, dtype=np.int16).reshape(-1, nr))
+            keep.append(ch)
+            descs[k].inward = int(bool(ix.inward))
+            descs[k].physical = int(bool(ix.physical))
+            descs[k].size = ch.shape[0]
+            descs[k].charges = _ptr(ch, C.c_int16)
+        p = np.ascontiguousarray(p, dtype=np.int64)
+        i = np.ascontiguousarray(i, dtype=np.int64)
+        xv = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).view(np.float64)
+        if i.size == 0:
+            i = np.zeros(1, np.int64)
+            xv = np.zeros(2, np.float64)
+        h = _p()
+        _check(_LIB.cam_tensor_from_csc(ctx.h, nr, _ptr(rules, C.c_int16), len(indices), descs, numrows,
+                                         rows, cols, _ptr(p, C.c_int64), _ptr(i, C.c_int64),
+                                         _ptr(xv, C.c_double), C.byref(h)))
+        return Tensor(ctx, h)
+
+    def shape(self):
+        n, r, nr = C.c_int(), C.c_int(), C.c_int()
+        _check(_LIB.cam_tensor_shape(self.h, C.byref(n), C.byref(r), C.byref(nr)))
+        return n.value, r.value, nr.value
+
+    def indices(self):
+        n, _, nr = self.shape()
+        out = []
+        for k in range(n):
+            inw, ph, sz = C.c_int32(), C.c_int32(), C.c_int64()
+            _check(_LIB.cam_tensor_index(self.h, k, C.byref(inw), C.byref(ph), C.byref(sz), None))
+            ch = np.zeros((sz.value, nr), np.int16)
+            _check(_LIB.cam_tensor_index(self.h, k, None, None, None, _ptr(ch, C.c_int16)))
+            out.append(IndexDesc(bool(inw.value), bool(ph.value), ch.astype(np.int64)))
+        return out
+
+    @property
+    def numrows(self):
+        return self.shape()[1]
+
+    def stored(self):
+        n = C.c_int64()
+        _check(_LIB.cam_tensor_nnz(self.h, C.byref(n)))
+        return n.value
+
+    def to_csc(self):
+        """Returns (rows, cols, p, i, x complex)."""
+        idx = self.indices()
+        nrow = self.numrows
+        rows = int(np.prod([ix.charges.shape[0] for ix in idx[:nrow]], dtype=np.int64))
+        cols = int(np.prod([ix.charges.shape[0] for ix in idx[nrow:]], dtype=np.int64))
+        nnz = C.c_int64()
+        _check(_LIB.cam_tensor_to_csc(self.h, 0, None, None, None, C.byref(nnz)))
+        cap = max(nnz.value, 1)
+        p = np.zeros(cols + 1, np.int64)
+        i = np.zeros(cap, np.int64)
+        x = np.zeros(2 * cap, np.float64)
+        _check(_LIB.cam_tensor_to_csc(self.h, cap, _ptr(p, C.c_int64), _ptr(i, C.c_int64),
+                                      _ptr(x, C.c_double), C.byref(nnz)))
+        n = nnz.value
+        return rows, cols, p, i[:n], x[:2 * n].view(np.complex128)
+
+    def shift_num_row_indices(self, n):
+        _check(_LIB.cam_tensor_shift_num_row_indices(self.h, n))
+        return self
+
+    def remove_dummy_indices(self, which):
+        w = np.ascontiguousarray(which, dtype=np.int64)
+        _check(_LIB.cam_tensor_remove_dummy_indices(self.h, len(w), _ptr(w, C.c_int64)))
+        return self
+
+    def move_dummy_indices(self, order):
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        _check(_LIB.cam_tensor_move_dummy_indices(self.h, len(o), _ptr(o, C.c_int64)))
+        return self
+
+    def rescale(self, factor):
+        f = complex(factor)
+        _check(_LIB.cam_tensor_rescale(self.h, f.real, f.imag))
+        return self
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_tensor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def contract(A: Tensor, conjA: bool, B: Tensor, conjB: bool, pairs) -> Tensor:
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int64).reshape(-1))
+    h = _p()
+    _check(_LIB.cam_contract(A.ctx.h, A.h, int(conjA), B.h, int(conjB), len(pr) // 2,
+                             _ptr(pr, C.c_int64), C.byref(h)))
+    return Tensor(A.ctx, h)
+
+
+def reorder(A: Tensor, conjA: bool, order, numrows) -> Tensor:
+    o = np.ascontiguousarray(order, dtype=np.int64)
+    h = _p()
+    _check(_LIB.cam_reorder(A.ctx.h, A.h, int(conjA), len(o), _ptr(o, C.c_int64), numrows, C.byref(h)))
+    return Tensor(A.ctx, h)
+
+
+class Heff:
+    """TwoVertexComponents on the device."""
+
+    def __init__(self, L: Tensor, W: Tensor, R: Tensor):
+        self.ctx = L.ctx
+        self._keep = (L, W, R)
+        h = _p()
+        n = C.c_int64()
+        _check(_LIB.cam_heff_create(L.ctx.h, L.h, W.h, R.h, C.byref(h), C.byref(n)))
+        self.h = h
+        self.n = n.value
+
+    def allowed(self):
+        out = np.zeros(max(self.n, 1), np.int64)
+        _check(_LIB.cam_heff_allowed_map(self.h, _ptr(out, C.c_int64)))
+        return out[:self.n]
+
+    @property
+    def flops(self):
+        f = C.c_double()
+        _check(_LIB.cam_heff_flops(self.h, C.byref(f)))
+        return f.value
+
+    def apply(self, vin, precision=CAM_FP64):
+        vin = np.ascontiguousarray(vin, dtype=np.complex128)
+        out = np.zeros(self.n, np.complex128)
+        _check(_LIB.cam_heff_apply(self.h, vin.ctypes.data, out.ctypes.data, CAM_MEM_HOST, precision))
+        return out
+
+    def apply_device(self, din, dout, precision=CAM_FP64, blocked=True):
+        """din/dout: device pointers (ints) of n complex128."""
+        if blocked:
+            _check(_LIB.cam_heff_apply_blocked(self.h, din, dout, precision))
+        else:
+            _check(_LIB.cam_heff_apply(self.h, din, dout, CAM_MEM_DEVICE, precision))
+
+    def to_blocked(self, din, dout):
+        _check(_LIB.cam_heff_to_blocked(self.h, din, dout))
+
+    def from_blocked(self, din, dout):
+        _check(_LIB.cam_heff_from_blocked(self.h, din, dout))
+
+    def lowest(self, guess=None, converge=True, tol=0.0, ncv=20, mixed=False):
+        """Returns (eigenvalue, eigenvector (reference order), info dict)."""
+        ev = C.c_double()
+        vec = np.zeros(self.n, np.complex128)
+        info = np.zeros(8, np.float64)
+        g = None
+        if guess is not None:
+            g = np.ascontiguousarray(guess, dtype=np.complex128)
+        _check(_LIB.cam_heff_lowest(self.h, None if g is None else g.ctypes.data, int(converge), tol, ncv,
+                                    int(mixed), C.byref(ev), vec.ctypes.data, _ptr(info, C.c_double)))
+        return ev.value, vec, dict(matvecs_fp64=int(info[0]), matvecs_fp32=int(info[1]),
+                                   restarts=int(info[2]), residual=float(info[3]))
+
+    def vjp(self, psi, nnzW):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        dpsi = np.zeros(self.n, np.complex128)
+        dW = np.zeros(max(nnzW, 1), np.complex128)
+        _check(_LIB.cam_heff_vjp(self.h, psi.ctypes.data, dpsi.ctypes.data, dW.ctypes.data, nnzW))
+        return dpsi, dW[:nnzW]
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_heff_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def svd_truncate(theta: Tensor, D=0, max_trunc=0.0):
+    """Returns (U, values, Vd, discarded)."""
+    ctx = theta.ctx
+    cap = 1 << 22
+    vals = np.zeros(cap, np.float64)
+    nv = C.c_int64()
+    disc = C.c_double()
+    U, V = _p(), _p()
+    _check(_LIB.cam_svd_truncate(ctx.h, theta.h, int(D), float(max_trunc), C.byref(U), C.byref(V),
+                                 _ptr(vals, C.c_double), cap, C.byref(nv), C.byref(disc)))
+    return Tensor(ctx, U), vals[:nv.value].copy(), Tensor(ctx, V), disc.value
+
+
+def tebd_gate(G: Tensor, A0: Tensor, A1: Tensor, lambda0, D=0, minS=0.0):
+    """Returns (A0new, A1new, lambda1, truncation)."""
+    ctx = G.ctx
+    l0 = np.ascontiguousarray(lambda0, dtype=np.float64)
+    cap = 1 << 22
+    l1 = np.zeros(cap, np.float64)
+    n1 = C.c_int64()
+    tr = C.c_double()
+    a, b = _p(), _p()
+    _check(_LIB.cam_tebd_gate(ctx.h, G.h, A0.h, A1.h, _ptr(l0, C.c_double), len(l0), int(D), float(minS),
+                              C.byref(a), C.byref(b), _ptr(l1, C.c_double), cap, C.byref(n1), C.byref(tr)))
+    return Tensor(ctx, a), Tensor(ctx, b), l1[:n1.value].copy(), tr.value

@@ -1,0 +1,393 @@
+// C ABI (include/chainamps_amd.h).  Every entry point converts exceptions
+// into a non-zero status plus cam_last_error() text.
+#include <cstring>
+#include <string>
+
+#include "../../include/chainamps_amd.h"
+#include "ctx.hpp"
+#include "heff.hpp"
+#include "solvers.hpp"
+
+using namespace cam;
+
+struct cam_ctx_s {
+  Ctx ctx;
+  explicit cam_ctx_s(int d) : ctx(d) {}
+};
+struct cam_tensor_s {
+  TensorPtr t;
+  cam_ctx* owner;
+};
+struct cam_heff_s {
+  Heff h;
+  cam_ctx* owner;
+  DevBuf<cplx> bin, bout, rin, rout;
+};
+
+namespace cam {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& s) { g_last_error = s; }
+const char* last_error() { return g_last_error.c_str(); }
+}  // namespace cam
+
+#define CAM_API_BEGIN try {
+#define CAM_API_END                                   \
+  return 0;                                           \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    cam::set_last_error(e.what());                    \
+    return 1;                                         \
+  }                                                   \
+  catch (...) {                                       \
+    cam::set_last_error("unknown error");             \
+    return 1;                                         \
+  }
+
+static void set_device(cam_ctx* c) { HIP_CHECK(hipSetDevice(c->ctx.device)); }
+
+extern "C" {
+
+const char* cam_last_error(void) { return cam::last_error(); }
+const char* cam_version(void) { return "chainamps_amd 0.1 (gfx950)"; }
+
+int cam_ctx_create(int device, cam_ctx** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(out, "null output");
+  int n = 0;
+  HIP_CHECK(hipGetDeviceCount(&n));
+  CAM_REQUIRE(device >= 0 && device < n, "no such HIP device");
+  *out = new cam_ctx(device);
+  CAM_API_END
+}
+
+int cam_ctx_destroy(cam_ctx* c) {
+  CAM_API_BEGIN
+  delete c;
+  CAM_API_END
+}
+
+int cam_ctx_sync(cam_ctx* c) {
+  CAM_API_BEGIN
+  HIP_CHECK(hipStreamSynchronize(c->ctx.stream));
+  CAM_API_END
+}
+
+int cam_ctx_stream(cam_ctx* c, void** stream) {
+  CAM_API_BEGIN
+  *stream = reinterpret_cast<void*>(c->ctx.stream);
+  CAM_API_END
+}
+
+int cam_tensor_from_csc(cam_ctx* c, int nrules, const int16_t* rules, int nidx,
+                        const cam_index_desc* idx, int num_row_indices, int64_t rows, int64_t cols,
+                        const int64_t* p, const int64_t* i, const double* x, cam_tensor** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(c && out && p, "null argument");
+  CAM_REQUIRE(nrules >= 1 && nrules <= kMaxRules, "unsupported number of charge rules");
+  CAM_REQUIRE(nidx >= 0 && nidx <= 16, "unsupported number of indices");
+  set_device(c);
+  Rules R;
+  R.N.assign(rules, rules + nrules);
+  std::vector<QIndex> qi(nidx);
+  for (int k = 0; k < nidx; ++k) {
+    qi[k].inward = idx[k].inward != 0;
+    qi[k].physical = idx[k].physical != 0;
+    qi[k].size = idx[k].size;
+    CAM_REQUIRE(idx[k].size >= 0, "negative index size");
+    qi[k].charges.resize(idx[k].size);
+    for (int64_t e = 0; e < idx[k].size; ++e) qi[k].charges[e] = R.normalise(idx[k].charges + e * nrules);
+  }
+  auto t = tensor_from_csc(R, std::move(qi), num_row_indices, rows, cols, p, i, x, c->ctx.stream);
+  *out = new cam_tensor{t, c};
+  CAM_API_END
+}
+
+int cam_tensor_shape(const cam_tensor* t, int* nidx, int* nrow, int* nrules) {
+  CAM_API_BEGIN
+  if (nidx) *nidx = t->t->nidx();
+  if (nrow) *nrow = t->t->numrows;
+  if (nrules) *nrules = t->t->rules.n();
+  CAM_API_END
+}
+
+int cam_tensor_index(const cam_tensor* t, int k, int32_t* inward, int32_t* physical, int64_t* size,
+                     int16_t* charges) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(k >= 0 && k < t->t->nidx(), "index out of range");
+  const QIndex& q = t->t->idx[k];
+  if (inward) *inward = q.inward;
+  if (physical) *physical = q.physical;
+  if (size) *size = q.size;
+  if (charges) {
+    const int nr = t->t->rules.n();
+    for (int64_t e = 0; e < q.size; ++e)
+      for (int r = 0; r < nr; ++r) charges[e * nr + r] = q.charges[e].q[r];
+  }
+  CAM_API_END
+}
+
+int cam_tensor_nnz(const cam_tensor* t, int64_t* nnz) {
+  CAM_API_BEGIN
+  *nnz = t->t->total;
+  CAM_API_END
+}
+
+int cam_tensor_to_csc(const cam_tensor* t, int64_t cap, int64_t* p, int64_t* i, double* x, int64_t* nnz) {
+  CAM_API_BEGIN
+  set_device(t->owner);
+  const int64_t n = tensor_to_csc(*t->t, cap, p, i, x, t->owner->ctx.stream);
+  if (nnz) *nnz = n;
+  CAM_REQUIRE(!p || n <= cap, "cam_tensor_to_csc: capacity too small");
+  CAM_API_END
+}
+
+int cam_tensor_destroy(cam_tensor* t) {
+  CAM_API_BEGIN
+  delete t;
+  CAM_API_END
+}
+
+int cam_tensor_shift_num_row_indices(cam_tensor* t, int nrow) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(nrow >= 1 && nrow <= t->t->nidx(), "Incorrect MPX reshape params for change rows");
+  t->t->numrows = nrow;
+  CAM_API_END
+}
+
+int cam_tensor_remove_dummy_indices(cam_tensor* tt, int n, const int64_t* which) {
+  CAM_API_BEGIN
+  BSTensor& t = *tt->t;
+  CAM_REQUIRE(n < t.nidx() - 1, "Too many indices for removal!");
+  std::vector<int> rm(t.nidx(), 0);
+  int nrow_removed = 0;
+  for (int k = 0; k < n; ++k) {
+    CAM_REQUIRE(which[k] >= 0 && which[k] < t.nidx() && t.idx[which[k]].size == 1, "Not a dummy index!");
+    rm[which[k]] = 1;
+  }
+  for (int k = 0; k < t.nidx(); ++k)
+    if (rm[k] && k < t.numrows) ++nrow_removed;
+  CAM_REQUIRE(nrow_removed < t.numrows, "All row indices would be removed, requiring a reshape!");
+  std::vector<QIndex> ni;
+  for (int k = 0; k < t.nidx(); ++k)
+    if (!rm[k]) ni.push_back(t.idx[k]);
+  for (Block& b : t.blocks) {
+    std::vector<int32_t> s;
+    std::vector<int64_t> d;
+    for (int k = 0; k < t.nidx(); ++k)
+      if (!rm[k]) { s.push_back(b.sec[k]); d.push_back(b.dims[k]); }
+    b.sec = s;
+    b.dims = d;
+  }
+  t.idx = ni;
+  t.numrows -= nrow_removed;
+  t.rebuild_lookup();
+  CAM_API_END
+}
+
+int cam_tensor_move_dummy_indices(cam_tensor* tt, int n, const int64_t* order) {
+  CAM_API_BEGIN
+  BSTensor& t = *tt->t;
+  CAM_REQUIRE(n == t.nidx(), "Incorrect MPX reshape params");
+  std::vector<int> nondummy;
+  std::vector<QIndex> ni;
+  int newrows = t.numrows;
+  for (int p = 0; p < n; ++p) {
+    const int o = static_cast<int>(order[p]);
+    CAM_REQUIRE(o >= 0 && o < n, "bad order");
+    if (t.idx[o].size > 1) nondummy.push_back(o);
+    if (o == t.numrows) newrows = p;
+    ni.push_back(t.idx[o]);
+  }
+  for (size_t k = 1; k < nondummy.size(); ++k)
+    CAM_REQUIRE(nondummy[k] > nondummy[k - 1], "Non dummy order shouldn't change!");
+  for (Block& b : t.blocks) {
+    std::vector<int32_t> s(n);
+    std::vector<int64_t> d(n);
+    for (int p = 0; p < n; ++p) { s[p] = b.sec[order[p]]; d[p] = b.dims[order[p]]; }
+    b.sec = s;
+    b.dims = d;
+  }
+  t.idx = ni;
+  t.numrows = newrows;
+  t.rebuild_lookup();
+  CAM_API_END
+}
+
+int cam_tensor_rescale(cam_tensor* t, double re, double im) {
+  CAM_API_BEGIN
+  set_device(t->owner);
+  launch_scale(cplx{re, im}, t->t->data.get(), t->t->total, t->owner->ctx.stream);
+  HIP_CHECK(hipStreamSynchronize(t->owner->ctx.stream));
+  CAM_API_END
+}
+
+int cam_contract(cam_ctx* c, const cam_tensor* A, int conjA, const cam_tensor* B, int conjB, int npairs,
+                 const int64_t* pairs, cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  std::vector<std::pair<int, int>> pr;
+  for (int k = 0; k < npairs; ++k)
+    pr.emplace_back(static_cast<int>(pairs[2 * k]), static_cast<int>(pairs[2 * k + 1]));
+  auto t = contract(c->ctx, *A->t, conjA != 0, *B->t, conjB != 0, pr);
+  *out = new cam_tensor{t, c};
+  CAM_API_END
+}
+
+int cam_reorder(cam_ctx* c, const cam_tensor* A, int conjA, int n, const int64_t* order, int nrow,
+                cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  std::vector<int> o(order, order + n);
+  auto t = reorder(c->ctx, *A->t, conjA != 0, o, nrow);
+  *out = new cam_tensor{t, c};
+  CAM_API_END
+}
+
+int cam_heff_create(cam_ctx* c, const cam_tensor* L, const cam_tensor* W, const cam_tensor* R,
+                    cam_heff** out, int64_t* n) {
+  CAM_API_BEGIN
+  set_device(c);
+  auto* h = new cam_heff;
+  h->owner = c;
+  try {
+    h->h.build(c->ctx, L->t, W->t, R->t);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  if (n) *n = h->h.n;
+  CAM_API_END
+}
+
+int cam_heff_destroy(cam_heff* h) {
+  CAM_API_BEGIN
+  delete h;
+  CAM_API_END
+}
+
+int cam_heff_allowed_map(const cam_heff* hc, int64_t* idx) {
+  CAM_API_BEGIN
+  auto* h = const_cast<cam_heff*>(hc);
+  set_device(h->owner);
+  h->h.build_perm();
+  std::memcpy(idx, h->h.allowed.data(), h->h.allowed.size() * sizeof(int64_t));
+  CAM_API_END
+}
+
+int cam_heff_flops(const cam_heff* h, double* flops) {
+  CAM_API_BEGIN
+  *flops = h->h.flops;
+  CAM_API_END
+}
+
+int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int precision) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  h->h.apply(reinterpret_cast<const cplx*>(in), reinterpret_cast<cplx*>(out), precision);
+  CAM_API_END
+}
+
+int cam_heff_to_blocked(cam_heff* h, const double* in_dev, double* out_dev) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  h->h.build_perm();
+  launch_scatter_map(reinterpret_cast<const cplx*>(in_dev), reinterpret_cast<cplx*>(out_dev),
+                     h->h.perm.get(), h->h.n, h->owner->ctx.stream);
+  CAM_API_END
+}
+
+int cam_heff_from_blocked(cam_heff* h, const double* in_dev, double* out_dev) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  h->h.build_perm();
+  launch_gather_map(reinterpret_cast<const cplx*>(in_dev), reinterpret_cast<cplx*>(out_dev),
+                    h->h.perm.get(), h->h.n, h->owner->ctx.stream);
+  CAM_API_END
+}
+
+int cam_heff_apply(cam_heff* h, const double* in, double* out, int mem_kind, int precision) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  CAM_REQUIRE(mem_kind == CAM_MEM_HOST || mem_kind == CAM_MEM_DEVICE, "bad mem_kind");
+  Heff& H = h->h;
+  hipStream_t s = h->owner->ctx.stream;
+  H.build_perm();
+  const int64_t n = H.n;
+  h->bin.ensure(std::max<int64_t>(n, 1));
+  h->bout.ensure(std::max<int64_t>(n, 1));
+  const cplx* rin = reinterpret_cast<const cplx*>(in);
+  cplx* rout = reinterpret_cast<cplx*>(out);
+  if (mem_kind == CAM_MEM_HOST) {
+    h->rin.ensure(std::max<int64_t>(n, 1));
+    h->rout.ensure(std::max<int64_t>(n, 1));
+    h->rin.upload(rin, n, s);
+    rin = h->rin.get();
+    rout = h->rout.get();
+  }
+  launch_scatter_map(rin, h->bin.get(), H.perm.get(), n, s);
+  H.apply(h->bin.get(), h->bout.get(), precision);
+  launch_gather_map(h->bout.get(), rout, H.perm.get(), n, s);
+  if (mem_kind == CAM_MEM_HOST) {
+    HIP_CHECK(hipMemcpyAsync(out, rout, n * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  CAM_API_END
+}
+
+int cam_heff_lowest(cam_heff* h, const double* guess, int converge, double tol, int ncv, int mixed,
+                    double* eval, double* evec, double* info) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  heff_lowest(h->h, reinterpret_cast<const cplx*>(guess), converge != 0, tol, ncv, mixed != 0, eval,
+              reinterpret_cast<cplx*>(evec), info);
+  CAM_API_END
+}
+
+int cam_svd_truncate(cam_ctx* c, const cam_tensor* theta, int64_t D, double max_trunc, cam_tensor** U,
+                     cam_tensor** Vd, double* values, int64_t cap, int64_t* nvals, double* discarded) {
+  CAM_API_BEGIN
+  set_device(c);
+  TensorPtr u, vd;
+  std::vector<double> vals;
+  double disc = 0.0;
+  svd_truncate(c->ctx, *theta->t, D, max_trunc, u, vd, vals, disc);
+  CAM_REQUIRE(static_cast<int64_t>(vals.size()) <= cap, "cam_svd_truncate: values capacity too small");
+  std::copy(vals.begin(), vals.end(), values);
+  if (nvals) *nvals = static_cast<int64_t>(vals.size());
+  if (discarded) *discarded = disc;
+  *U = new cam_tensor{u, c};
+  *Vd = new cam_tensor{vd, c};
+  CAM_API_END
+}
+
+int cam_tebd_gate(cam_ctx* c, const cam_tensor* G, const cam_tensor* A0, const cam_tensor* A1,
+                  const double* lambda0, int64_t n0, int64_t D, double minS, cam_tensor** A0n,
+                  cam_tensor** A1n, double* lambda1, int64_t cap, int64_t* n1, double* truncation) {
+  CAM_API_BEGIN
+  set_device(c);
+  TensorPtr a0, a1;
+  std::vector<double> l1;
+  double tr = 0.0;
+  std::vector<double> l0(lambda0, lambda0 + n0);
+  tebd_gate(c->ctx, *G->t, *A0->t, *A1->t, l0, D, minS, a0, a1, l1, tr);
+  CAM_REQUIRE(static_cast<int64_t>(l1.size()) <= cap, "cam_tebd_gate: lambda capacity too small");
+  std::copy(l1.begin(), l1.end(), lambda1);
+  if (n1) *n1 = static_cast<int64_t>(l1.size());
+  if (truncation) *truncation = tr;
+  *A0n = new cam_tensor{a0, c};
+  *A1n = new cam_tensor{a1, c};
+  CAM_API_END
+}
+
+int cam_heff_vjp(cam_heff* h, const double* psi, double* dpsi, double* dW, int64_t nnzW) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  heff_vjp(h->h, reinterpret_cast<const cplx*>(psi), reinterpret_cast<cplx*>(dpsi),
+           reinterpret_cast<cplx*>(dW), nnzW);
+  CAM_API_END
+}
+
+}  // extern "C"

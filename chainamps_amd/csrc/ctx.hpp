@@ -1,0 +1,36 @@
+// Library context: one device, one stream, reusable scratch.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "bstensor.hpp"
+#include "common.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+namespace cam {
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* blas = nullptr;          // rocblas_handle (created lazily)
+  DevBuf<cplx> scratchA, scratchB;
+  DevBuf<PermJob> perm_jobs;
+  DevBuf<int64_t> perm_chunks;
+  explicit Ctx(int dev);
+  ~Ctx();
+  void* blas_handle();
+};
+
+bool indices_match(const QIndex& a, bool conjA, const QIndex& b, bool conjB);
+
+TensorPtr contract(Ctx& ctx, const BSTensor& A, bool conjA, const BSTensor& B, bool conjB,
+                   const std::vector<std::pair<int, int>>& pairs);
+TensorPtr reorder(Ctx& ctx, const BSTensor& A, bool conjA, const std::vector<int>& order,
+                  int numrows);
+// Permute every block of A into `out` (already laid out with blocks in the
+// permuted sector order) -- used by reorder and contract.
+void permute_tensor(Ctx& ctx, const BSTensor& A, const std::vector<int>& order, bool conj,
+                    const BSTensor& out_layout, cplx* out);
+
+}  // namespace cam

@@ -1,0 +1,387 @@
+// Grouped persistent complex GEMM kernels (gfx950 MFMA).
+//
+// Workgroup = 256 threads = 4 wave64s arranged 2x2; a TM x TN output tile
+// per workgroup, each wave owning a (TM/2) x (TN/2) sub-tile made of 16x16
+// MFMA fragments.  K is consumed in chunks of 16 staged through LDS with
+// real and imaginary parts in separate planes (so one MFMA operand is one
+// LDS read); the next chunk is prefetched into registers while the current
+// one is multiplied.  A complex product is four real MFMAs:
+//   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre.
+//
+// MFMA 16x16x4 operand layout (wave64): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator element v of lane l is D[4*(l/16)+v][l%16].
+#include <algorithm>
+#include <numeric>
+
+#include "gemm.hpp"
+
+namespace cam {
+
+namespace {
+
+constexpr int KC = 16;     // K chunk staged in LDS
+constexpr int PADL = 4;    // LDS row padding (elements)
+
+template <typename R> struct CT;
+template <> struct CT<double> { using T = double2; };
+template <> struct CT<float> { using T = float2; };
+
+template <typename R> struct Mf;
+template <> struct Mf<double> {
+  typedef double acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mf<float> {
+  typedef float acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename R, int TM, int TN>
+__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __restrict__ probs,
+                                                           const GemmSeg* __restrict__ segs,
+                                                           const GemmTile* __restrict__ tiles,
+                                                           int ntiles, GemmBases bases) {
+  using C = typename CT<R>::T;
+  using acc_t = typename Mf<R>::acc;
+  constexpr int LDA = TM + PADL;
+  constexpr int LDB = TN + PADL;
+  constexpr int PA = TM * KC / 256;   // A elements per thread per chunk
+  constexpr int PB = TN * KC / 256;   // B elements per thread per chunk
+  constexpr int FM = TM / 32;         // 16-row fragments per wave
+  constexpr int FN = TN / 32;
+  __shared__ R sAr[KC * LDA], sAi[KC * LDA], sBr[KC * LDB], sBi[KC * LDB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int fr = lane & 15, fk = lane >> 4;
+
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tl = tiles[t];
+    const GemmProb P = probs[tl.prob];
+    const int M = P.M, N = P.N;
+    const int m0 = tl.m0, n0 = tl.n0;
+
+    acc_t accr[FM][FN], acci[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        accr[i][j] = acc_t{0, 0, 0, 0};
+        acci[i][j] = acc_t{0, 0, 0, 0};
+      }
+
+    C ra[PA], rb[PB];
+    const int send = P.seg0 + P.nseg;
+    int s = P.seg0;
+    while (s < send && segs[s].K <= 0) ++s;
+    int k0 = 0;
+    GemmSeg S;
+    if (s < send) S = segs[s];
+
+    auto load = [&](const GemmSeg& G, int kk0) {
+      const int asp = (G.flags >> 8) & 15, bsp = (G.flags >> 12) & 15;
+      const C* A = (asp ? reinterpret_cast<const C*>(bases.p[asp]) + reinterpret_cast<intptr_t>(G.A)
+                        : reinterpret_cast<const C*>(G.A)) +
+                   static_cast<int64_t>(tl.batch) * G.bsA;
+      const C* B = (bsp ? reinterpret_cast<const C*>(bases.p[bsp]) + reinterpret_cast<intptr_t>(G.B)
+                        : reinterpret_cast<const C*>(G.B)) +
+                   static_cast<int64_t>(tl.batch) * G.bsB;
+#pragma unroll
+      for (int p = 0; p < PA; ++p) {
+        const int e = tid + p * 256;
+        const int m = e % TM, k = e / TM;
+        const int gm = m0 + m, gk = kk0 + k;
+        C v;
+        v.x = 0; v.y = 0;
+        if (gm < M && gk < G.K) v = A[gm + static_cast<int64_t>(gk) * G.csA];
+        ra[p] = v;
+      }
+      if (G.flags & kSegBNContig) {
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+          const int e = tid + p * 256;
+          const int n = e % TN, k = e / TN;
+          const int gn = n0 + n, gk = kk0 + k;
+          C v;
+          v.x = 0; v.y = 0;
+          if (gn < N && gk < G.K) v = B[static_cast<int64_t>(gk) * G.ldB + gn];
+          rb[p] = v;
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+          const int e = tid + p * 256;
+          const int k = e % KC, n = e / KC;
+          const int gn = n0 + n, gk = kk0 + k;
+          C v;
+          v.x = 0; v.y = 0;
+          if (gn < N && gk < G.K) v = B[gk + static_cast<int64_t>(gn) * G.ldB];
+          rb[p] = v;
+        }
+      }
+    };
+    auto store = [&](const GemmSeg& G) {
+      const bool cA = G.flags & kSegConjA, cB = G.flags & kSegConjB, al = G.flags & kSegAlpha;
+      const R are = static_cast<R>(G.alpha_re), aim = static_cast<R>(G.alpha_im);
+#pragma unroll
+      for (int p = 0; p < PA; ++p) {
+        const int e = tid + p * 256;
+        const int m = e % TM, k = e / TM;
+        R xr = ra[p].x, xi = cA ? -ra[p].y : ra[p].y;
+        if (al) {
+          const R tr = xr * are - xi * aim;
+          xi = xr * aim + xi * are;
+          xr = tr;
+        }
+        sAr[k * LDA + m] = xr;
+        sAi[k * LDA + m] = xi;
+      }
+      if (G.flags & kSegBNContig) {
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+          const int e = tid + p * 256;
+          const int n = e % TN, k = e / TN;
+          sBr[k * LDB + n] = rb[p].x;
+          sBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+          const int e = tid + p * 256;
+          const int k = e % KC, n = e / KC;
+          sBr[k * LDB + n] = rb[p].x;
+          sBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
+        }
+      }
+    };
+
+    if (s < send) load(S, 0);
+    while (s < send) {
+      __syncthreads();
+      store(S);
+      __syncthreads();
+      // advance (segment, k0) and prefetch the next chunk
+      int ns = s, nk = k0 + KC;
+      GemmSeg NS = S;
+      if (nk >= S.K) {
+        nk = 0;
+        ++ns;
+        while (ns < send && segs[ns].K <= 0) ++ns;
+        if (ns < send) NS = segs[ns];
+      }
+      if (ns < send) load(NS, nk);
+#pragma unroll
+      for (int ks = 0; ks < KC / 4; ++ks) {
+        const int kr = ks * 4 + fk;
+        R ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          ar[i] = sAr[kr * LDA + wm * (TM / 2) + i * 16 + fr];
+          ai[i] = sAi[kr * LDA + wm * (TM / 2) + i * 16 + fr];
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          br[j] = sBr[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+          bi[j] = sBi[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            accr[i][j] = Mf<R>::mma(ar[i], br[j], accr[i][j]);
+            accr[i][j] = Mf<R>::mma(-ai[i], bi[j], accr[i][j]);
+            acci[i][j] = Mf<R>::mma(ar[i], bi[j], acci[i][j]);
+            acci[i][j] = Mf<R>::mma(ai[i], br[j], acci[i][j]);
+          }
+      }
+      s = ns;
+      k0 = nk;
+      S = NS;
+    }
+
+    // epilogue
+    C* Cp = (P.cspace ? reinterpret_cast<C*>(const_cast<void*>(bases.p[P.cspace])) +
+                            reinterpret_cast<intptr_t>(P.C)
+                      : reinterpret_cast<C*>(P.C)) +
+            static_cast<int64_t>(tl.batch) * P.bsC;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int n = n0 + wn * (TN / 2) + j * 16 + fr;
+        if (n >= N) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + wm * (TM / 2) + i * 16 + 4 * fk + v;
+          if (m < M) {
+            C* dst = Cp + m + static_cast<int64_t>(n) * P.csC;
+            C val;
+            val.x = accr[i][j][v];
+            val.y = acci[i][j][v];
+            if (P.beta) {
+              const C old = *dst;
+              val.x += old.x;
+              val.y += old.y;
+            }
+            *dst = val;
+          }
+        }
+      }
+    __syncthreads();
+  }
+}
+
+template <typename R, int TM, int TN>
+void launch_shape(const GemmPlan& plan, int shape, hipStream_t s, int grid_limit,
+                  const GemmBases& bases) {
+  const int nt = static_cast<int>(plan.tiles[shape].size());
+  if (!nt) return;
+  static int cus = device_cu_count();
+  int grid = std::min(nt, cus * 4);
+  if (grid_limit > 0) grid = std::min(grid, grid_limit);
+  hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN>), dim3(grid), dim3(256), 0, s,
+                     plan.d_probs.get(), plan.d_segs.get(), plan.d_tiles[shape].get(), nt, bases);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+int device_cu_count() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  return prop.multiProcessorCount;
+}
+
+int GemmPlan::add_problem(void* C, int64_t csC, int M, int N, int batch, int64_t bsC, int beta,
+                          int cspace) {
+  GemmProb p{};
+  p.C = C;
+  p.csC = csC;
+  p.bsC = bsC;
+  p.M = M;
+  p.N = N;
+  p.seg0 = static_cast<int32_t>(segs.size());
+  p.nseg = 0;
+  p.batch = batch;
+  p.beta = beta;
+  p.tile = 0;
+  p.cspace = cspace;
+  probs.push_back(p);
+  uploaded = false;
+  return static_cast<int>(probs.size()) - 1;
+}
+
+void GemmPlan::add_segment(int prob, const void* A, int64_t csA, int64_t bsA, const void* B,
+                           int64_t ldB, int64_t bsB, int K, int flags, double are, double aim) {
+  GemmProb& p = probs[prob];
+  CAM_REQUIRE(p.seg0 + p.nseg == static_cast<int32_t>(segs.size()),
+              "segments of a problem must be added contiguously");
+  GemmSeg g{};
+  g.A = A;
+  g.B = B;
+  g.csA = csA;
+  g.ldB = ldB;
+  g.bsA = bsA;
+  g.bsB = bsB;
+  g.K = K;
+  g.flags = flags;
+  g.alpha_re = are;
+  g.alpha_im = aim;
+  if (are != 1.0 || aim != 0.0) g.flags |= kSegAlpha;
+  segs.push_back(g);
+  p.nseg++;
+  flops += 8.0 * double(p.M) * double(p.N) * double(K) * double(p.batch);
+  uploaded = false;
+}
+
+void GemmPlan::choose_tiles(int mode) {
+  for (auto& p : probs) {
+    if (mode == 0) { p.tile = 0; continue; }
+    double best = 1e300;
+    int bi = 0;
+    for (int t = 0; t < kNumTileShapes; ++t) {
+      const double tm = kTileShapes[t].tm, tn = kTileShapes[t].tn;
+      const double pm = std::ceil(p.M / tm) * tm, pn = std::ceil(p.N / tn) * tn;
+      // padded area, with a small per-tile overhead so bigger tiles win ties
+      const double cost = pm * pn * (1.0 + 0.02 * (4096.0 / (tm * tn)));
+      if (cost < best) { best = cost; bi = t; }
+    }
+    p.tile = bi;
+  }
+  uploaded = false;
+}
+
+void GemmPlan::build_tiles() {
+  for (int t = 0; t < kNumTileShapes; ++t) tiles[t].clear();
+  std::vector<double> cost[kNumTileShapes];
+  for (size_t pi = 0; pi < probs.size(); ++pi) {
+    const GemmProb& p = probs[pi];
+    int64_t ktot = 0;
+    for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) ktot += (segs[s].K + KC - 1) / KC * KC;
+    if (ktot == 0 && p.beta) continue;   // nothing to add
+    const int tm = kTileShapes[p.tile].tm, tn = kTileShapes[p.tile].tn;
+    for (int b = 0; b < p.batch; ++b)
+      for (int n0 = 0; n0 < p.N; n0 += tn)
+        for (int m0 = 0; m0 < p.M; m0 += tm) {
+          tiles[p.tile].push_back(GemmTile{static_cast<int32_t>(pi), b, m0, n0});
+          cost[p.tile].push_back(double(ktot) + 8.0);
+        }
+  }
+  // longest first (LPT) for the persistent round-robin
+  for (int t = 0; t < kNumTileShapes; ++t) {
+    std::vector<size_t> ord(tiles[t].size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[t][a] > cost[t][b]; });
+    std::vector<GemmTile> sorted;
+    sorted.reserve(ord.size());
+    for (auto o : ord) sorted.push_back(tiles[t][o]);
+    tiles[t].swap(sorted);
+  }
+  uploaded = false;
+}
+
+void GemmPlan::upload(hipStream_t s) {
+  d_probs.ensure(std::max<size_t>(probs.size(), 1));
+  d_probs.upload(probs.data(), probs.size(), s);
+  d_segs.ensure(std::max<size_t>(segs.size(), 1));
+  d_segs.upload(segs.data(), segs.size(), s);
+  for (int t = 0; t < kNumTileShapes; ++t) {
+    d_tiles[t].ensure(std::max<size_t>(tiles[t].size(), 1));
+    d_tiles[t].upload(tiles[t].data(), tiles[t].size(), s);
+  }
+  uploaded = true;
+}
+
+void GemmPlan::clear() {
+  probs.clear();
+  segs.clear();
+  for (auto& t : tiles) t.clear();
+  flops = 0.0;
+  uploaded = false;
+}
+
+void launch_grouped_gemm(GemmPlan& plan, int precision, hipStream_t s, const GemmBases* bases,
+                         int grid_limit) {
+  if (!plan.uploaded) plan.upload(s);
+  GemmBases b{};
+  if (bases) b = *bases;
+  if (precision == 0) {
+    launch_shape<double, 64, 64>(plan, 0, s, grid_limit, b);
+    launch_shape<double, 32, 64>(plan, 1, s, grid_limit, b);
+    launch_shape<double, 64, 32>(plan, 2, s, grid_limit, b);
+  } else {
+    launch_shape<float, 64, 64>(plan, 0, s, grid_limit, b);
+    launch_shape<float, 32, 64>(plan, 1, s, grid_limit, b);
+    launch_shape<float, 64, 32>(plan, 2, s, grid_limit, b);
+  }
+}
+
+}  // namespace cam

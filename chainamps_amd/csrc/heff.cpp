@@ -1,0 +1,381 @@
+// H_eff planner and matvec (see heff.hpp for the stage definitions).
+//
+// Input conventions are those of the reference's TwoVertexComponents
+// (DMRG_routines.cpp:779-833): the left block L has indices
+// [dummy(out), a0'(in), dummy(in), bL(out), dummy(in), a0(out)], the MPO W
+// [sigma'(in), bl(in), sigma(out), br(out)] and the right block R
+// [a2'(out), dummy(in), bR(in), dummy(out), a2(in), dummy(out)].  In block
+// storage the size-1 dummies vanish from the layout, so the L block of
+// sectors (s0', sb, s0) is the array L[a0', b, a0] (a0' fastest) and the
+// matrix L_b(a0', a0) for one bond value b is a strided view of it.
+#include "heff.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace cam {
+
+namespace {
+
+struct EnvMat {            // strided view of L_b or R_b for one sector pair
+  int sout;                // output sector (a0' or a2')
+  int64_t off;             // element offset in the tensor data
+  int64_t ld;              // m_out * nb   (column stride over the inner index)
+  int64_t mout, min;
+};
+
+// (b value, input sector) -> view; input sector = s0 (L) or s3 (R).
+std::map<std::pair<int64_t, int>, EnvMat> env_views(const BSTensor& T, int iout, int ib, int iin,
+                                                    const std::vector<cplx>& host, bool have_host) {
+  std::map<std::pair<int64_t, int>, EnvMat> out;
+  for (const Block& bl : T.blocks) {
+    const int so = bl.sec[iout], sb = bl.sec[ib], si = bl.sec[iin];
+    const int64_t mo = bl.dims[iout], nb = bl.dims[ib], mi = bl.dims[iin];
+    const auto& bpos = T.idx[ib].sec_pos[sb];
+    for (int64_t j = 0; j < nb; ++j) {
+      // skip exactly-zero sub-blocks (e.g. the dummy left block selects one b)
+      if (have_host) {
+        bool nz = false;
+        for (int64_t k = 0; k < mi && !nz; ++k)
+          for (int64_t i = 0; i < mo; ++i) {
+            const cplx v = host[bl.offset + i + mo * (j + nb * k)];
+            if (v.re != 0.0 || v.im != 0.0) { nz = true; break; }
+          }
+        if (!nz) continue;
+      }
+      EnvMat e{so, bl.offset + mo * j, mo * nb, mo, mi};
+      out[{bpos[j], si}] = e;
+    }
+  }
+  return out;
+}
+
+struct WMat {             // W(s', bl, s, br) for fixed (bl, br) and sectors
+  int64_t off, ld;        // W^T(k=s, n=s') at off + n + k*ld
+  int64_t nout, nin;
+};
+
+}  // namespace
+
+void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
+  ctx = &c;
+  L = L_;
+  W = W_;
+  R = R_;
+  CAM_REQUIRE(L->nidx() == 6 && R->nidx() == 6 && W->nidx() == 4,
+              "heff: expected 6-index blocks and a 4-index MPO");
+  CAM_REQUIRE(L->rules == W->rules && R->rules == W->rules, "Physical Indices don't match!");
+  rules = W->rules;
+  // the contractions the reference performs (contract13, contract32, and
+  // the psi pairs 30,41 / 20,31,42) must be legal
+  CAM_REQUIRE(indices_match(W->idx[1], false, L->idx[3], false) || indices_match(L->idx[3], false, W->idx[1], false),
+              "heff: MPO left bond does not match the left block");
+  CAM_REQUIRE(indices_match(W->idx[3], false, R->idx[2], false),
+              "heff: MPO right bond does not match the right block");
+  for (int k : {0, 2, 4}) CAM_REQUIRE(L->idx[k].size == 1, "heff: left block dummy index not size 1");
+  for (int k : {1, 3, 5}) CAM_REQUIRE(R->idx[k].size == 1, "heff: right block dummy index not size 1");
+  CAM_REQUIRE(L->idx[1].same_charges(L->idx[5]), "heff: left block bra/ket bonds differ");
+  CAM_REQUIRE(R->idx[0].same_charges(R->idx[4]), "heff: right block bra/ket bonds differ");
+  CAM_REQUIRE(W->idx[0].same_charges(W->idx[2]), "heff: MPO physical indices differ");
+  sig = W->idx[2];
+  a0 = L->idx[5];
+  a2 = R->idx[4];
+  d = sig.size;
+  chiL = a0.size;
+  chiR = a2.size;
+  w = W->idx[1].size;
+  hipStream_t s = ctx->stream;
+
+  // ---- psi blocks ---------------------------------------------------------
+  pblocks.clear();
+  pindex.clear();
+  n = 0;
+  for (int s3 = 0; s3 < a2.nsec(); ++s3)
+    for (int s2 = 0; s2 < sig.nsec(); ++s2)
+      for (int s1 = 0; s1 < sig.nsec(); ++s1)
+        for (int s0 = 0; s0 < a0.nsec(); ++s0) {
+          const Charge q = rules.add(rules.add(sig.sec_q[s1], a0.sec_q[s0]), sig.sec_q[s2]);
+          if (q != a2.sec_q[s3]) continue;
+          PB b{s0, s3, s1, s2, a0.sec_size[s0], a2.sec_size[s3], sig.sec_size[s1], sig.sec_size[s2], n, 0};
+          b.size = b.m0 * b.m3 * b.n1 * b.n2;
+          pindex[{s0, s3, s1, s2}] = static_cast<int>(pblocks.size());
+          pblocks.push_back(b);
+          n += b.size;
+        }
+
+  // ---- operand views ---------------------------------------------------------
+  const int64_t kHostCheckLimit = int64_t(1) << 24;  // elements
+  std::vector<cplx> hL, hR;
+  const bool chkL = L->total <= kHostCheckLimit, chkR = R->total <= kHostCheckLimit;
+  if (chkL) hL = download(*L, s);
+  if (chkR) hR = download(*R, s);
+  auto Lv = env_views(*L, 1, 3, 5, hL, chkL);   // (b, s0) -> a0' view
+  auto Rv = env_views(*R, 0, 2, 4, hR, chkR);   // (b, s3) -> a2' view
+  std::vector<cplx> hW = download(*W, s);
+  // W views: key (bl, br, s', s)
+  std::map<std::array<int64_t, 4>, WMat> Wv;
+  for (const Block& bl : W->blocks) {
+    const int sp = bl.sec[0], sbl = bl.sec[1], ss = bl.sec[2], sbr = bl.sec[3];
+    const int64_t no = bl.dims[0], nbl = bl.dims[1], ni = bl.dims[2], nbr = bl.dims[3];
+    for (int64_t jl = 0; jl < nbl; ++jl)
+      for (int64_t jr = 0; jr < nbr; ++jr) {
+        const int64_t base = bl.offset + no * jl + no * nbl * ni * jr;
+        bool nz = false;
+        for (int64_t k = 0; k < ni && !nz; ++k)
+          for (int64_t i = 0; i < no; ++i) {
+            const cplx v = hW[base + i + k * no * nbl];
+            if (v.re != 0.0 || v.im != 0.0) { nz = true; break; }
+          }
+        if (!nz) continue;
+        const int64_t bl_pos = W->idx[1].sec_pos[sbl][jl];
+        const int64_t br_pos = W->idx[3].sec_pos[sbr][jr];
+        Wv[{bl_pos, br_pos, sp, ss}] = WMat{base, no * nbl, no, ni};
+      }
+  }
+
+  // ---- intermediates ------------------------------------------------------------
+  t1.clear(); t2.clear(); t3.clear();
+  t1n = t2n = t3n = 0;
+  struct SegA { int pb; EnvMat L; };
+  std::map<std::array<int, 5>, SegA> segA;
+  for (size_t pi = 0; pi < pblocks.size(); ++pi) {
+    const PB& P = pblocks[pi];
+    for (int64_t b = 0; b < w; ++b) {
+      auto it = Lv.find({b, P.s0});
+      if (it == Lv.end()) continue;
+      const EnvMat& e = it->second;
+      std::array<int, 5> key{static_cast<int>(b), e.sout, P.s3, P.s1, P.s2};
+      TB tb{t1n, e.mout, P.m3, P.n1, P.n2};
+      t1[key] = tb;
+      t1n += tb.m0 * tb.m3 * tb.n1 * tb.n2;
+      segA[key] = SegA{static_cast<int>(pi), e};
+    }
+  }
+  // stage B: T2[b1](s0',s3,s1',s2) <- T1[bL](s0',s3,s1,s2) x W(s1',bL,s1,b1)
+  struct SegW { std::array<int, 5> src; WMat W; };
+  std::map<std::array<int, 5>, std::vector<SegW>> segB, segC;
+  for (auto& kv : t1) {
+    const auto& k = kv.first;  // (bL, s0', s3, s1, s2)
+    for (auto& wv : Wv) {
+      const auto& wk = wv.first;  // (bl, br, s', s)
+      if (wk[0] != k[0] || wk[3] != k[3]) continue;
+      std::array<int, 5> okey{static_cast<int>(wk[1]), k[1], k[2], static_cast<int>(wk[2]), k[4]};
+      segB[okey].push_back(SegW{k, wv.second});
+    }
+  }
+  for (auto& kv : segB) {
+    const auto& k = kv.first;
+    const TB& src = t1[kv.second.front().src];
+    TB tb{t2n, src.m0, src.m3, sig.sec_size[k[3]], src.n2};
+    t2[k] = tb;
+    t2n += tb.m0 * tb.m3 * tb.n1 * tb.n2;
+  }
+  // stage C: T3[bR](s0',s3,s1',s2') <- T2[b1](s0',s3,s1',s2) x W(s2',b1,s2,bR)
+  for (auto& kv : t2) {
+    const auto& k = kv.first;  // (b1, s0', s3, s1', s2)
+    for (auto& wv : Wv) {
+      const auto& wk = wv.first;
+      if (wk[0] != k[0] || wk[3] != k[4]) continue;
+      std::array<int, 5> okey{static_cast<int>(wk[1]), k[1], k[2], k[3], static_cast<int>(wk[2])};
+      segC[okey].push_back(SegW{k, wv.second});
+    }
+  }
+  for (auto& kv : segC) {
+    const auto& k = kv.first;
+    const TB& src = t2[kv.second.front().src];
+    TB tb{t3n, src.m0, src.m3, src.n1, sig.sec_size[k[4]]};
+    t3[k] = tb;
+    t3n += tb.m0 * tb.m3 * tb.n1 * tb.n2;
+  }
+  // stage D: out(s0',s3',s1',s2') <- T3[bR](s0',s3,s1',s2') x R_bR(s3'<-s3)
+  struct SegR { std::array<int, 5> src; EnvMat R; };
+  std::map<int, std::vector<SegR>> segD;   // output psi block -> segments
+  for (auto& kv : t3) {
+    const auto& k = kv.first;  // (bR, s0', s3, s1', s2')
+    auto it = Rv.find({k[0], k[2]});
+    if (it == Rv.end()) continue;
+    const EnvMat& e = it->second;
+    auto pit = pindex.find({k[1], e.sout, k[3], k[4]});
+    CAM_REQUIRE(pit != pindex.end(), "heff: output block outside the allowed space (flux)");
+    segD[pit->second].push_back(SegR{k, e});
+  }
+
+  T1.alloc(std::max<int64_t>(t1n, 1));
+  T2.alloc(std::max<int64_t>(t2n, 1));
+  T3.alloc(std::max<int64_t>(t3n, 1));
+
+  // ---- plans: operands addressed as (space, element offset), so the same
+  // plan runs on the FP64 tensors and on their FP32 copies ----------------
+  flops = 0;
+  for (int i = 0; i < 4; ++i) {
+    st[i].clear();
+    stage_flops[i] = 0;
+  }
+  auto off = [](int64_t o) { return reinterpret_cast<const void*>(o); };
+  auto offw = [](int64_t o) { return reinterpret_cast<void*>(o); };
+  for (auto& kv : segA) {
+    const TB& tb = t1.at(kv.first);
+    const PB& Pb = pblocks[kv.second.pb];
+    const EnvMat& e = kv.second.L;
+    const int N = static_cast<int>(Pb.m3 * Pb.n1 * Pb.n2);
+    const int pi = st[0].add_problem(offw(tb.off), tb.m0, static_cast<int>(tb.m0), N, 1, 0, 0, kSpT1);
+    st[0].add_segment(pi, off(e.off), e.ld, 0, off(Pb.off), Pb.m0, 0, static_cast<int>(Pb.m0),
+                      seg_spaces(kSpL, kSpIn));
+  }
+  for (auto& kv : segB) {
+    const TB& tb = t2.at(kv.first);
+    const int64_t rows = tb.m0 * tb.m3;
+    const int pi = st[1].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n1),
+                                     static_cast<int>(tb.n2), rows * tb.n1, 0, kSpT2);
+    for (const SegW& sg : kv.second) {
+      const TB& sb = t1.at(sg.src);
+      st[1].add_segment(pi, off(sb.off), rows, rows * sb.n1, off(sg.W.off), sg.W.ld, 0,
+                        static_cast<int>(sb.n1), kSegBNContig | seg_spaces(kSpT1, kSpW));
+    }
+  }
+  for (auto& kv : segC) {
+    const TB& tb = t3.at(kv.first);
+    const int64_t rows = tb.m0 * tb.m3 * tb.n1;
+    const int pi = st[2].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n2),
+                                     1, 0, 0, kSpT3);
+    for (const SegW& sg : kv.second) {
+      const TB& sb = t2.at(sg.src);
+      st[2].add_segment(pi, off(sb.off), rows, 0, off(sg.W.off), sg.W.ld, 0, static_cast<int>(sb.n2),
+                        kSegBNContig | seg_spaces(kSpT2, kSpW));
+    }
+  }
+  for (auto& kv : segD) {
+    const PB& Ob = pblocks[kv.first];
+    const int pi = st[3].add_problem(offw(Ob.off), Ob.m0, static_cast<int>(Ob.m0), static_cast<int>(Ob.m3),
+                                     static_cast<int>(Ob.n1 * Ob.n2), Ob.m0 * Ob.m3, 0, kSpOut);
+    for (const SegR& sg : kv.second) {
+      const TB& sb = t3.at(sg.src);
+      st[3].add_segment(pi, off(sb.off), sb.m0, sb.m0 * sb.m3, off(sg.R.off), sg.R.ld, 0,
+                        static_cast<int>(sb.m3), kSegBNContig | seg_spaces(kSpT3, kSpR));
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    st[i].choose_tiles(1);
+    st[i].build_tiles();
+    st[i].upload(ctx->stream);
+    stage_flops[i] = st[i].flops;
+    flops += st[i].flops;
+  }
+  // output blocks that receive nothing must be zeroed after the matvec
+  untouched.clear();
+  for (size_t pi = 0; pi < pblocks.size(); ++pi)
+    if (!segD.count(static_cast<int>(pi))) untouched.push_back(static_cast<int>(pi));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+GemmBases Heff::bases(int precision, const void* in, void* out) const {
+  GemmBases b{};
+  if (precision == 0) {
+    b.p[kSpL] = L->data.get();
+    b.p[kSpW] = W->data.get();
+    b.p[kSpR] = R->data.get();
+    b.p[kSpT1] = T1.get();
+    b.p[kSpT2] = T2.get();
+    b.p[kSpT3] = T3.get();
+  } else {
+    b.p[kSpL] = L32.get();
+    b.p[kSpW] = W32.get();
+    b.p[kSpR] = R32.get();
+    b.p[kSpT1] = T1f.get();
+    b.p[kSpT2] = T2f.get();
+    b.p[kSpT3] = T3f.get();
+  }
+  b.p[kSpIn] = in;
+  b.p[kSpOut] = out;
+  return b;
+}
+
+void Heff::prepare_fp32() {
+  if (fp32_ready) return;
+  hipStream_t s = ctx->stream;
+  L32.alloc(std::max<int64_t>(L->total, 1));
+  W32.alloc(std::max<int64_t>(W->total, 1));
+  R32.alloc(std::max<int64_t>(R->total, 1));
+  launch_to_fp32(L->data.get(), L32.get(), L->total, s);
+  launch_to_fp32(W->data.get(), W32.get(), W->total, s);
+  launch_to_fp32(R->data.get(), R32.get(), R->total, s);
+  T1f.alloc(std::max<int64_t>(t1n, 1));
+  T2f.alloc(std::max<int64_t>(t2n, 1));
+  T3f.alloc(std::max<int64_t>(t3n, 1));
+  vinf.alloc(std::max<int64_t>(n, 1));
+  voutf.alloc(std::max<int64_t>(n, 1));
+  HIP_CHECK(hipStreamSynchronize(s));
+  fp32_ready = true;
+}
+
+void Heff::apply_blocked(const cplx* in, cplx* out) {
+  hipStream_t s = ctx->stream;
+  const GemmBases b = bases(0, in, out);
+  for (int i = 0; i < 4; ++i) launch_grouped_gemm(st[i], 0, s, &b);
+  for (int pi : untouched) launch_zero(out + pblocks[pi].off, pblocks[pi].size * sizeof(cplx), s);
+}
+
+void Heff::apply_blocked32(const float2* in, float2* out) {
+  prepare_fp32();
+  hipStream_t s = ctx->stream;
+  const GemmBases b = bases(1, in, out);
+  for (int i = 0; i < 4; ++i) launch_grouped_gemm(st[i], 1, s, &b);
+  for (int pi : untouched) launch_zero(out + pblocks[pi].off, pblocks[pi].size * sizeof(float2), s);
+}
+
+void Heff::apply(const cplx* in, cplx* out, int precision) {
+  if (precision == 0) {
+    apply_blocked(in, out);
+    return;
+  }
+  prepare_fp32();
+  hipStream_t s = ctx->stream;
+  launch_to_fp32(in, vinf.get(), n, s);
+  apply_blocked32(vinf.get(), voutf.get());
+  launch_to_fp64(voutf.get(), out, n, s);
+}
+
+void Heff::build_perm() {
+  if (perm_ready) return;
+  // reference compressed order (DMRG_routines.cpp:808-824): loop a2 > s2 >
+  // a0 > s1, keep entries with sigma1 == a2 - sigma2 - a0; flat index
+  // s1 + d*a0 + d*chiL*(s2 + d*a2) is ascending in that loop order.
+  allowed.clear();
+  allowed.reserve(static_cast<size_t>(n));
+  std::vector<int64_t> pm;
+  pm.reserve(static_cast<size_t>(n));
+  for (int64_t ia2 = 0; ia2 < chiR; ++ia2) {
+    const int s3 = a2.pos_sec[ia2];
+    const int64_t o3 = a2.pos_off[ia2];
+    for (int64_t is2 = 0; is2 < d; ++is2) {
+      const int s2 = sig.pos_sec[is2];
+      const int64_t o2 = sig.pos_off[is2];
+      const Charge acc2 = rules.sub(a2.charges[ia2], sig.charges[is2]);
+      for (int64_t ia0 = 0; ia0 < chiL; ++ia0) {
+        const Charge need = rules.sub(acc2, a0.charges[ia0]);
+        const int s1 = sig.find_sector(need);
+        if (s1 < 0) continue;
+        const int s0 = a0.pos_sec[ia0];
+        const int64_t o0 = a0.pos_off[ia0];
+        auto it = pindex.find({s0, s3, s1, s2});
+        CAM_REQUIRE(it != pindex.end(), "heff: allowed entry without a block");
+        const PB& P = pblocks[it->second];
+        const int64_t colbase = d * chiL * (is2 + d * ia2) + d * ia0;
+        for (int64_t o1 = 0; o1 < P.n1; ++o1) {
+          const int64_t is1 = sig.sec_pos[s1][o1];
+          allowed.push_back(is1 + colbase);
+          pm.push_back(P.off + o0 + P.m0 * (o3 + P.m3 * (o1 + P.n1 * o2)));
+        }
+      }
+    }
+  }
+  CAM_REQUIRE(static_cast<int64_t>(allowed.size()) == n, "heff: allowed count mismatch");
+  // rows with the same (a2,s2,a0) were emitted in sector order == ascending
+  // position order, so `allowed` is ascending
+  perm.alloc(std::max<int64_t>(n, 1));
+  perm.upload(pm.data(), pm.size(), ctx->stream);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  perm_ready = true;
+}
+
+}  // namespace cam

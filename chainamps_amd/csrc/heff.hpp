@@ -1,0 +1,66 @@
+// Two-vertex effective Hamiltonian H_eff = L . W . W . R on QN blocks.
+//
+// Vector layout ("block layout"): psi[sigma1, a0, sigma2, a2] is stored as
+// blocks keyed (s0, s3, s1, s2) -- sectors of a0, a2, sigma1, sigma2 with
+// q(s1)+q(s0)+q(s2) == q(s3) -- each block dense over (a0, a2, sigma1,
+// sigma2), a0 fastest.  The matvec is four grouped MFMA launches:
+//   A: T1[bL](a0',a2,s1,s2)  = sum_a0       L_bL(a0',a0)         psi
+//   B: T2[b1](a0',a2,s1',s2) = sum_{bL,s1}  W(s1',bL,s1,b1)      T1
+//   C: T3[bR](a0',a2,s1',s2')= sum_{b1,s2}  W(s2',b1,s2,bR)      T2
+//   D: out(a0',a2',s1',s2')  = sum_{bR,a2}  R_bR(a2',a2)         T3
+// (the optimal order; the reference's LeftPart/RightPart order costs a
+// factor ~d more, DMRG_routines.cpp:779-833, 898-974).
+#pragma once
+#include <array>
+#include <map>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace cam {
+
+// operand spaces of the stage plans
+enum HeffSpace { kSpL = 1, kSpW, kSpR, kSpT1, kSpT2, kSpT3, kSpIn, kSpOut };
+
+struct Heff {
+  Ctx* ctx = nullptr;
+  TensorPtr L, W, R;       // keep the inputs alive (shared)
+  Rules rules;
+  QIndex sig, a0, a2;      // sigma (W idx 2), a0 (L idx 5), a2 (R idx 4)
+  int64_t d = 0, chiL = 0, chiR = 0, w = 0;
+
+  struct PB { int s0, s3, s1, s2; int64_t m0, m3, n1, n2, off, size; };
+  std::vector<PB> pblocks;
+  std::map<std::array<int, 4>, int> pindex;   // (s0,s3,s1,s2) -> block
+  int64_t n = 0;
+
+  // intermediates: key (b, s0', s3, sx, sy) -> offset; sizes
+  struct TB { int64_t off, m0, m3, n1, n2; };
+  std::map<std::array<int, 5>, TB> t1, t2, t3;
+  int64_t t1n = 0, t2n = 0, t3n = 0;
+
+  DevBuf<cplx> T1, T2, T3, vin, vout;          // FP64
+  DevBuf<float2> L32, W32, R32, T1f, T2f, T3f, vinf, voutf;  // FP32
+  bool fp32_ready = false;
+  GemmPlan st[4];
+  std::vector<int> untouched;   // psi blocks no stage-D problem writes
+  double flops = 0.0;
+  double stage_flops[4] = {0, 0, 0, 0};
+
+  // reference compressed order map (built on demand)
+  std::vector<int64_t> allowed;   // flat positions
+  DevBuf<int64_t> perm;           // perm[i] = block-layout offset of allowed[i]
+  bool perm_ready = false;
+
+  void build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_);
+  void build_perm();
+  void prepare_fp32();
+  GemmBases bases(int precision, const void* in, void* out) const;
+  // device block-layout vectors
+  void apply_blocked(const cplx* in, cplx* out);
+  void apply_blocked32(const float2* in, float2* out);
+  // precision: 0 fp64, 1 fp32 (input/output FP64 block layout)
+  void apply(const cplx* in, cplx* out, int precision);
+};
+
+}  // namespace cam

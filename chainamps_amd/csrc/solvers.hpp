@@ -1,0 +1,26 @@
+// Eigensolver, SVD truncation, iTEBD gate and adjoint on the device.
+#pragma once
+#include <vector>
+
+#include "ctx.hpp"
+#include "heff.hpp"
+
+namespace cam {
+
+// Lowest eigenpair of H_eff (replaces arpack_eigs at TwoVertexComponents::HED).
+void heff_lowest(Heff& H, const cplx* guess_ref, bool converge, double tol, int ncv, bool mixed,
+                 double* eval, cplx* evec_ref, double* info);
+
+// MPX_matrix::SVD with the reference truncation rule.
+void svd_truncate(Ctx& ctx, const BSTensor& theta, int64_t D, double max_trunc, TensorPtr& U,
+                  TensorPtr& Vd, std::vector<double>& values, double& discarded);
+
+// iTEBD::apply_and_decompose.
+void tebd_gate(Ctx& ctx, const BSTensor& G, const BSTensor& A0, const BSTensor& A1,
+               const std::vector<double>& lambda0, int64_t D, double minS, TensorPtr& A0n,
+               TensorPtr& A1n, std::vector<double>& lambda1, double& truncation);
+
+// dE/dconj(psi) and dE/dW for E = <psi|H|psi>/<psi|psi>.
+void heff_vjp(Heff& H, const cplx* psi_ref, cplx* dpsi_ref, cplx* dW, int64_t nnzW);
+
+}  // namespace cam

@@ -1,0 +1,169 @@
+/*
+ * chainamps_amd.h -- C ABI of the MI355X-native ChainAMPS hot path.
+ *
+ * The library keeps QN-blocked tensors resident on one MI355X (gfx950) and
+ * runs the two-vertex effective-Hamiltonian matvec, its Lanczos eigensolve,
+ * the blockwise SVD truncation and the iTEBD gate application with
+ * hand-written HIP kernels.  Every entry point returns 0 on success and a
+ * non-zero status otherwise; cam_last_error() gives the message (per
+ * thread).  Plain pointers and sizes only; complex numbers are interleaved
+ * (re, im) doubles exactly as std::complex<double>.
+ *
+ * Tensor conventions follow the reference MPX_matrix: an ordered list of
+ * directed indices, the first `num_row_indices` of them forming the rows
+ * of a CSC matrix in mixed radix (first index fastest), the rest the
+ * columns (MPX_matrix.hpp:61-66, sparse_interface.cpp:85-91).
+ *
+ * Threading: one context per host thread; a context is not re-entrant.
+ * Calls taking host buffers synchronise before returning.
+ */
+#ifndef CHAINAMPS_AMD_H
+#define CHAINAMPS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cam_ctx_s cam_ctx;
+typedef struct cam_tensor_s cam_tensor;
+typedef struct cam_heff_s cam_heff;
+
+/* One tensor index: MPXIndex (MPXIndex.hpp:17-65).  `charges` holds
+ * size*nrules shorts, entry-major. */
+typedef struct {
+  int32_t inward;          /* 1 = Ingoing(), 0 = Outgoing() */
+  int32_t physical;        /* 1 = physical (model basis) index */
+  int64_t size;
+  const int16_t* charges;
+} cam_index_desc;
+
+enum { CAM_MEM_HOST = 0, CAM_MEM_DEVICE = 1 };
+enum { CAM_FP64 = 0, CAM_FP32 = 1 };
+
+/* ---- context --------------------------------------------------------
+ * Replaces nothing in the reference (which is single-process CPU code); a
+ * context owns one HIP stream, a rocBLAS/rocSOLVER handle and scratch. */
+int cam_ctx_create(int device, cam_ctx** out);
+int cam_ctx_destroy(cam_ctx* ctx);
+int cam_ctx_sync(cam_ctx* ctx);
+/* The hipStream_t the context launches on (for events/profiling). */
+int cam_ctx_stream(cam_ctx* ctx, void** stream);
+const char* cam_last_error(void);
+/* Library build/version string. */
+const char* cam_version(void);
+
+/* ---- tensors (MPX_matrix storage, MPX_matrix.cpp:29-35) ------------------
+ * Upload a CSC tensor; the zero-flux rule IngoingRowState(r) ==
+ * OutgoingColState(c) (MPX_matrix.cpp:275-302) is enforced: a violating
+ * entry is an error. `rules` are the Z_N charge rules (states.hpp:32-34). */
+int cam_tensor_from_csc(cam_ctx* ctx, int nrules, const int16_t* rules,
+                        int nidx, const cam_index_desc* idx, int num_row_indices,
+                        int64_t rows, int64_t cols, const int64_t* p,
+                        const int64_t* i, const double* x, cam_tensor** out);
+/* Number of indices / row indices, and one index descriptor (charges
+ * copied to `charges` when non-NULL). */
+int cam_tensor_shape(const cam_tensor* t, int* nidx, int* num_row_indices, int* nrules);
+int cam_tensor_index(const cam_tensor* t, int k, int32_t* inward, int32_t* physical,
+                     int64_t* size, int16_t* charges);
+/* Stored block-sparse element count (upper bound of the CSC nnz). */
+int cam_tensor_nnz(const cam_tensor* t, int64_t* nnz);
+/* Download as CSC (exact zeros dropped); p has cols+1 entries, i and x
+ * capacity `cap` entries; *nnz receives the count. */
+int cam_tensor_to_csc(const cam_tensor* t, int64_t cap, int64_t* p, int64_t* i,
+                      double* x, int64_t* nnz);
+int cam_tensor_destroy(cam_tensor* t);
+/* MPX_matrix::ShiftNumRowIndices (MPX_matrix.cpp:72-85) -- metadata only
+ * in block storage. */
+int cam_tensor_shift_num_row_indices(cam_tensor* t, int num_row_indices);
+/* MPX_matrix::RemoveDummyIndices (MPX_matrix.cpp:415-457). */
+int cam_tensor_remove_dummy_indices(cam_tensor* t, int n, const int64_t* which);
+/* MPX_matrix::MoveDummyIndices (MPX_matrix.cpp:87-105). */
+int cam_tensor_move_dummy_indices(cam_tensor* t, int n, const int64_t* order);
+/* Scale all values (MPX_matrix::Rescale, MPX_matrix.cpp:459-462). */
+int cam_tensor_rescale(cam_tensor* t, double re, double im);
+
+/* ---- contraction ------------------------------------------------------
+ * ajaj::contract (MPX_matrix.hpp:25, MPX_matrix.cpp:744-791): output
+ * indices are A's kept indices (flipped if conjA) then B's kept indices
+ * (flipped if conjB); num_row_indices = #kept A.  `pairs` holds npairs
+ * (a,b) index pairs.  One grouped MFMA launch over all QN-compatible block
+ * pairs. */
+int cam_contract(cam_ctx* ctx, const cam_tensor* A, int conjA, const cam_tensor* B,
+                 int conjB, int npairs, const int64_t* pairs, cam_tensor** out);
+/* ajaj::reorder (MPX_matrix.cpp:630-638). */
+int cam_reorder(cam_ctx* ctx, const cam_tensor* A, int conjA, int n,
+                const int64_t* order, int num_row_indices, cam_tensor** out);
+
+/* ---- two-vertex effective Hamiltonian --------------------------------
+ * TwoVertexComponents (DMRG_routines.hpp:189-214, DMRG_routines.cpp:779-833)
+ * from the left block L (6 indices), the MPO W (4) and the right block R (6).
+ * *n receives the length of the QN-allowed compressed vector. */
+int cam_heff_create(cam_ctx* ctx, const cam_tensor* L, const cam_tensor* W,
+                    const cam_tensor* R, cam_heff** out, int64_t* n);
+int cam_heff_destroy(cam_heff* h);
+/* allowed_indices of DMRG_routines.cpp:808-824 (loop order a2>s2>a0>s1):
+ * n flat positions into the d*chiL*d*chiR vector. */
+int cam_heff_allowed_map(const cam_heff* h, int64_t* idx);
+/* Algorithmic flop count of one matvec (sum of 8*m*n*k over the
+ * QN-allowed block triples of the four-stage order L, W, W, R). */
+int cam_heff_flops(const cam_heff* h, double* flops);
+/* TwoVertexMPOMPSMultiply (DMRG_routines.cpp:898-974): out = H_eff . in,
+ * in/out of length n in the reference compressed order.  mem_kind selects
+ * host or device buffers; precision CAM_FP64 or CAM_FP32 (split-complex
+ * MFMA, FP32 accumulation). */
+int cam_heff_apply(cam_heff* h, const double* in, double* out, int mem_kind, int precision);
+/* Same matvec on vectors already in the device block layout (the Lanczos
+ * layout), no gather/scatter; `in`/`out` device pointers of n complex. */
+int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int precision);
+/* Reference order <-> device block layout (K3 gather/scatter). */
+int cam_heff_to_blocked(cam_heff* h, const double* in_dev, double* out_dev);
+int cam_heff_from_blocked(cam_heff* h, const double* in_dev, double* out_dev);
+
+/* Lowest eigenpair ("SR"), replacing arpack_eigs<TwoVertexComponents,...>
+ * (arpack_interface.hpp:85-111, 174-237) at TwoVertexComponents::HED
+ * (DMRG_routines.cpp:835-896).  guess: host vector (reference order) or
+ * NULL.  converge=0 gives the reference's growth-step setting (tol 0.1);
+ * converge=1 tolerance `tol` (<=0: machine precision).  ncv: Krylov size
+ * (reference 20).  mixed=1 runs the bulk in FP32 with FP64 residual
+ * correction and final FP64 steps.  evec: host, reference order, phase
+ * gauge of DMRG_routines.cpp:870-890 applied.  info (may be NULL) receives
+ * {matvecs_fp64, matvecs_fp32, restarts, residual_norm}. */
+int cam_heff_lowest(cam_heff* h, const double* guess, int converge, double tol,
+                    int ncv, int mixed, double* eval, double* evec, double* info);
+
+/* ---- blockwise SVD truncation -------------------------------------------
+ * MPX_matrix::SVD -> SparseMatrix::SVD (MPX_matrix.cpp:157-192,
+ * sparse_interface.cpp:1248-1406): per QN column block SVD in FP64 on the
+ * GPU, global descending sort (ties by block then index), keep
+ * min(D, #values) (D=0: all), tail truncation below max_trunc, degeneracy
+ * extension (relative gap < 1e-3), U/V^dagger entries below eps/rows
+ * dropped, new bond charges from U's rows.  values has capacity cap;
+ * *nvals receives the kept count; *discarded the discarded weight. */
+int cam_svd_truncate(cam_ctx* ctx, const cam_tensor* theta, int64_t D, double max_trunc,
+                     cam_tensor** U, cam_tensor** Vd, double* values, int64_t cap,
+                     int64_t* nvals, double* discarded);
+
+/* ---- iTEBD gate ------------------------------------------------------------
+ * iTEBD::apply_and_decompose (TEBD_routines.cpp:233-266): A0, A1 left-shape
+ * MPS tensors, G the 4-index bond operator, lambda0 (n0 values) the bond
+ * weights on A0's left bond.  Produces the new A0, A1 (swapped as in the
+ * reference), lambda1 (capacity cap) and the truncation. */
+int cam_tebd_gate(cam_ctx* ctx, const cam_tensor* G, const cam_tensor* A0,
+                  const cam_tensor* A1, const double* lambda0, int64_t n0, int64_t D,
+                  double minS, cam_tensor** A0n, cam_tensor** A1n, double* lambda1,
+                  int64_t cap, int64_t* n1, double* truncation);
+
+/* ---- adjoint (no counterpart in the reference) --------------------------
+ * Vector-Jacobian product of E = <psi|H_eff|psi>/<psi|psi>:
+ * dpsi (host, n complex, reference order) = dE/dconj(psi) and
+ * dW (host, complex, same CSC positions as W's stored entries in
+ * cam_tensor_to_csc order, count nnzW) = dE/dW.  Environments are
+ * recomputed, never stored per Lanczos step. */
+int cam_heff_vjp(cam_heff* h, const double* psi, double* dpsi, double* dW, int64_t nnzW);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
