@@ -71,6 +71,7 @@ _SIGS = {
     "cam_heff_flops": [_p, _dp],
     "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
+    "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
     "cam_heff_to_blocked": [_p, _p, _p],
     "cam_heff_from_blocked": [_p, _p, _p],
     "cam_heff_lowest": [_p, _p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _p, _dp],
@@ -301,6 +302,14 @@ class Heff:
         else:
             _check(_LIB.cam_heff_apply(self.h, din, dout, CAM_MEM_DEVICE, precision))
 
+    def profile(self, precision=CAM_FP64, reps=5):
+        """Average ms per stage (L, W, W, R) from HIP events and the
+        per-stage algorithmic flops."""
+        ms = np.zeros(4)
+        fl = np.zeros(4)
+        _check(_LIB.cam_heff_profile(self.h, precision, reps, _ptr(ms, C.c_double), _ptr(fl, C.c_double)))
+        return ms, fl
+
     def to_blocked(self, din, dout):
         _check(_LIB.cam_heff_to_blocked(self.h, din, dout))
 
@@ -364,3 +373,55 @@ def tebd_gate(G: Tensor, A0: Tensor, A1: Tensor, lambda0, D=0, minS=0.0):
     _check(_LIB.cam_tebd_gate(ctx.h, G.h, A0.h, A1.h, _ptr(l0, C.c_double), len(l0), int(D), float(minS),
                               C.byref(a), C.byref(b), _ptr(l1, C.c_double), cap, C.byref(n1), C.byref(tr)))
     return Tensor(ctx, a), Tensor(ctx, b), l1[:n1.value].copy(), tr.value
+
+
+_SIGS_IDMRG = {
+    "cam_idmrg_create": [_p, _p, C.POINTER(C.c_int16), C.c_int, C.c_int, C.c_int, C.POINTER(_p)],
+    "cam_idmrg_grow": [_p, C.c_int64, C.c_double, _dp],
+    "cam_idmrg_values": [_p, _dp, C.c_int64, _i64p],
+    "cam_idmrg_destroy": [_p],
+}
+_SIGS.update(_SIGS_IDMRG)
+
+IDMRG_FIELDS = ("N", "energy", "energy_per_vertex", "entropy", "truncation", "chi", "n",
+                "matvecs_fp64", "matvecs_fp32", "residual", "seconds", "heff_flops")
+
+
+class IDMRG:
+    """Device iDMRG growth (iDMRG::run / SuperBlock::grow_two_vertex)."""
+
+    def __init__(self, W: Tensor, target=None, converge=False, mixed=False, ncv=20):
+        self.ctx = W.ctx
+        self._W = W
+        nr = W.shape()[2]
+        t = np.zeros(nr, np.int16) if target is None else np.ascontiguousarray(target, dtype=np.int16)
+        h = _p()
+        _check(_LIB.cam_idmrg_create(W.ctx.h, W.h, _ptr(t, C.c_int16), int(converge), int(mixed), ncv,
+                                     C.byref(h)))
+        self.h = h
+
+    def grow(self, chi, truncation=0.0):
+        rec = np.zeros(13)
+        _check(_LIB.cam_idmrg_grow(self.h, int(chi), float(truncation), _ptr(rec, C.c_double)))
+        d = dict(zip(IDMRG_FIELDS, rec.tolist()))
+        for k in ("N", "chi", "n", "matvecs_fp64", "matvecs_fp32"):
+            d[k] = int(d[k])
+        return d
+
+    def values(self):
+        n = C.c_int64()
+        _check(_LIB.cam_idmrg_values(self.h, None, 0, C.byref(n)))
+        v = np.zeros(max(n.value, 1))
+        _check(_LIB.cam_idmrg_values(self.h, _ptr(v, C.c_double), len(v), C.byref(n)))
+        return v[:n.value]
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_idmrg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,6 +24,62 @@ void BSTensor::finalize_layout() {
   data.alloc(static_cast<size_t>(total));
 }
 
+void BSTensor::remove_dummy_indices(const std::vector<int>& which) {
+  CAM_REQUIRE(static_cast<int>(which.size()) < nidx() - 1, "Too many indices for removal!");
+  std::vector<int> rm(nidx(), 0);
+  for (int w : which) {
+    CAM_REQUIRE(w >= 0 && w < nidx() && idx[w].size == 1, "Not a dummy index!");
+    rm[w] = 1;
+  }
+  int nrow_removed = 0;
+  for (int k = 0; k < nidx(); ++k)
+    if (rm[k] && k < numrows) ++nrow_removed;
+  CAM_REQUIRE(nrow_removed < numrows, "All row indices would be removed, requiring a reshape!");
+  std::vector<QIndex> ni;
+  for (int k = 0; k < nidx(); ++k)
+    if (!rm[k]) ni.push_back(idx[k]);
+  for (Block& b : blocks) {
+    std::vector<int32_t> sc;
+    std::vector<int64_t> d;
+    for (int k = 0; k < nidx(); ++k)
+      if (!rm[k]) { sc.push_back(b.sec[k]); d.push_back(b.dims[k]); }
+    b.sec = sc;
+    b.dims = d;
+  }
+  idx = ni;
+  numrows -= nrow_removed;
+  rebuild_lookup();
+}
+
+void BSTensor::move_dummy_indices(const std::vector<int>& order) {
+  const int n = nidx();
+  CAM_REQUIRE(static_cast<int>(order.size()) == n, "Incorrect MPX reshape params");
+  std::vector<int> nondummy;
+  std::vector<QIndex> ni;
+  int newrows = numrows;
+  std::vector<int> seen(n, 0);
+  for (int p = 0; p < n; ++p) {
+    const int o = order[p];
+    CAM_REQUIRE(o >= 0 && o < n && !seen[o], "bad order");
+    seen[o] = 1;
+    if (idx[o].size > 1) nondummy.push_back(o);
+    if (o == numrows) newrows = p;
+    ni.push_back(idx[o]);
+  }
+  for (size_t k = 1; k < nondummy.size(); ++k)
+    CAM_REQUIRE(nondummy[k] > nondummy[k - 1], "Non dummy order shouldn't change!");
+  for (Block& b : blocks) {
+    std::vector<int32_t> sc(n);
+    std::vector<int64_t> d(n);
+    for (int p = 0; p < n; ++p) { sc[p] = b.sec[order[p]]; d[p] = b.dims[order[p]]; }
+    b.sec = sc;
+    b.dims = d;
+  }
+  idx = ni;
+  numrows = newrows;
+  rebuild_lookup();
+}
+
 std::vector<std::vector<int32_t>> BSTensor::allowed_sector_tuples() const {
   std::vector<std::vector<int32_t>> out;
   int n = nidx();

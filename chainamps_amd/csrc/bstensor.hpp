@@ -72,6 +72,11 @@ struct BSTensor {
   // allocate device memory (uninitialised).
   void finalize_layout();
   void rebuild_lookup();
+  // Metadata-only index operations (no data moves in block storage):
+  // RemoveDummyIndices (MPX_matrix.cpp:415-457) and MoveDummyIndices
+  // (MPX_matrix.cpp:87-105).
+  void remove_dummy_indices(const std::vector<int>& which);
+  void move_dummy_indices(const std::vector<int>& order);
   // Enumerate every zero-flux sector tuple (used for dense-in-QN outputs).
   std::vector<std::vector<int32_t>> allowed_sector_tuples() const;
 };

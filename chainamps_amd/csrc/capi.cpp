@@ -7,6 +7,8 @@
 #include "ctx.hpp"
 #include "heff.hpp"
 #include "solvers.hpp"
+#include "idmrg.hpp"
+#include <memory>
 
 using namespace cam;
 
@@ -156,60 +158,13 @@ int cam_tensor_shift_num_row_indices(cam_tensor* t, int nrow) {
 
 int cam_tensor_remove_dummy_indices(cam_tensor* tt, int n, const int64_t* which) {
   CAM_API_BEGIN
-  BSTensor& t = *tt->t;
-  CAM_REQUIRE(n < t.nidx() - 1, "Too many indices for removal!");
-  std::vector<int> rm(t.nidx(), 0);
-  int nrow_removed = 0;
-  for (int k = 0; k < n; ++k) {
-    CAM_REQUIRE(which[k] >= 0 && which[k] < t.nidx() && t.idx[which[k]].size == 1, "Not a dummy index!");
-    rm[which[k]] = 1;
-  }
-  for (int k = 0; k < t.nidx(); ++k)
-    if (rm[k] && k < t.numrows) ++nrow_removed;
-  CAM_REQUIRE(nrow_removed < t.numrows, "All row indices would be removed, requiring a reshape!");
-  std::vector<QIndex> ni;
-  for (int k = 0; k < t.nidx(); ++k)
-    if (!rm[k]) ni.push_back(t.idx[k]);
-  for (Block& b : t.blocks) {
-    std::vector<int32_t> s;
-    std::vector<int64_t> d;
-    for (int k = 0; k < t.nidx(); ++k)
-      if (!rm[k]) { s.push_back(b.sec[k]); d.push_back(b.dims[k]); }
-    b.sec = s;
-    b.dims = d;
-  }
-  t.idx = ni;
-  t.numrows -= nrow_removed;
-  t.rebuild_lookup();
+  tt->t->remove_dummy_indices(std::vector<int>(which, which + n));
   CAM_API_END
 }
 
 int cam_tensor_move_dummy_indices(cam_tensor* tt, int n, const int64_t* order) {
   CAM_API_BEGIN
-  BSTensor& t = *tt->t;
-  CAM_REQUIRE(n == t.nidx(), "Incorrect MPX reshape params");
-  std::vector<int> nondummy;
-  std::vector<QIndex> ni;
-  int newrows = t.numrows;
-  for (int p = 0; p < n; ++p) {
-    const int o = static_cast<int>(order[p]);
-    CAM_REQUIRE(o >= 0 && o < n, "bad order");
-    if (t.idx[o].size > 1) nondummy.push_back(o);
-    if (o == t.numrows) newrows = p;
-    ni.push_back(t.idx[o]);
-  }
-  for (size_t k = 1; k < nondummy.size(); ++k)
-    CAM_REQUIRE(nondummy[k] > nondummy[k - 1], "Non dummy order shouldn't change!");
-  for (Block& b : t.blocks) {
-    std::vector<int32_t> s(n);
-    std::vector<int64_t> d(n);
-    for (int p = 0; p < n; ++p) { s[p] = b.sec[order[p]]; d[p] = b.dims[order[p]]; }
-    b.sec = s;
-    b.dims = d;
-  }
-  t.idx = ni;
-  t.numrows = newrows;
-  t.rebuild_lookup();
+  tt->t->move_dummy_indices(std::vector<int>(order, order + n));
   CAM_API_END
 }
 
@@ -286,6 +241,16 @@ int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int preci
   set_device(h->owner);
   CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
   h->h.apply(reinterpret_cast<const cplx*>(in), reinterpret_cast<cplx*>(out), precision);
+  CAM_API_END
+}
+
+int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, double* stage_flops) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  h->h.profile(precision, reps, stage_ms);
+  if (stage_flops)
+    for (int i = 0; i < 4; ++i) stage_flops[i] = h->h.stage_flops[i];
   CAM_API_END
 }
 
@@ -387,6 +352,52 @@ int cam_heff_vjp(cam_heff* h, const double* psi, double* dpsi, double* dW, int64
   set_device(h->owner);
   heff_vjp(h->h, reinterpret_cast<const cplx*>(psi), reinterpret_cast<cplx*>(dpsi),
            reinterpret_cast<cplx*>(dW), nnzW);
+  CAM_API_END
+}
+
+}  // extern "C"
+
+struct cam_idmrg_s {
+  std::unique_ptr<cam::IDMRG> d;
+  cam_ctx* owner;
+};
+
+extern "C" {
+
+int cam_idmrg_create(cam_ctx* c, const cam_tensor* W, const int16_t* target, int converge, int mixed, int ncv,
+                     cam_idmrg** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  Charge t = W->t->rules.normalise(target);
+  auto* r = new cam_idmrg;
+  r->owner = c;
+  r->d.reset(new cam::IDMRG(c->ctx, W->t, t, converge != 0, mixed != 0, ncv > 0 ? ncv : 20));
+  *out = r;
+  CAM_API_END
+}
+
+int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record) {
+  CAM_API_BEGIN
+  set_device(d->owner);
+  const cam::IDMRGRecord r = d->d->grow(chi, truncation);
+  const double v[13] = {double(r.N), r.energy, r.energy_per_vertex, r.entropy, r.truncation, double(r.chi),
+                        double(r.n), r.matvecs_fp64, r.matvecs_fp32, r.residual, r.seconds, r.heff_flops, 0.0};
+  std::copy(v, v + 13, record);
+  CAM_API_END
+}
+
+int cam_idmrg_values(const cam_idmrg* d, double* values, int64_t cap, int64_t* n) {
+  CAM_API_BEGIN
+  const auto& v = d->d->values;
+  if (n) *n = static_cast<int64_t>(v.size());
+  CAM_REQUIRE(static_cast<int64_t>(v.size()) <= cap || !values, "capacity too small");
+  if (values) std::copy(v.begin(), v.end(), values);
+  CAM_API_END
+}
+
+int cam_idmrg_destroy(cam_idmrg* d) {
+  CAM_API_BEGIN
+  delete d;
   CAM_API_END
 }
 

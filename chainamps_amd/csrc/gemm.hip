@@ -9,7 +9,9 @@
 //   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre.
 //
 // MFMA 16x16x4 operand layout (wave64): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; accumulator element v of lane l is D[4*(l/16)+v][l%16].
+// B[l/16][l%16].  Accumulator element v of lane l is D[row][l%16] with
+// row = l/16 + 4v for the f64 instruction and row = 4(l/16) + v for f32
+// (measured on gfx950).
 #include <algorithm>
 #include <numeric>
 
@@ -32,12 +34,14 @@ template <> struct Mf<double> {
   static __device__ __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int fk, int v) { return fk + 4 * v; }
 };
 template <> struct Mf<float> {
   typedef float acc __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int fk, int v) { return 4 * fk + v; }
 };
 
 template <typename R, int TM, int TN>
@@ -92,15 +96,28 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       const C* B = (bsp ? reinterpret_cast<const C*>(bases.p[bsp]) + reinterpret_cast<intptr_t>(G.B)
                         : reinterpret_cast<const C*>(G.B)) +
                    static_cast<int64_t>(tl.batch) * G.bsB;
+      if (G.flags & kSegAKContig) {
 #pragma unroll
-      for (int p = 0; p < PA; ++p) {
-        const int e = tid + p * 256;
-        const int m = e % TM, k = e / TM;
-        const int gm = m0 + m, gk = kk0 + k;
-        C v;
-        v.x = 0; v.y = 0;
-        if (gm < M && gk < G.K) v = A[gm + static_cast<int64_t>(gk) * G.csA];
-        ra[p] = v;
+        for (int p = 0; p < PA; ++p) {
+          const int e = tid + p * 256;
+          const int k = e % KC, m = e / KC;
+          const int gm = m0 + m, gk = kk0 + k;
+          C v;
+          v.x = 0; v.y = 0;
+          if (gm < M && gk < G.K) v = A[gk + static_cast<int64_t>(gm) * G.csA];
+          ra[p] = v;
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < PA; ++p) {
+          const int e = tid + p * 256;
+          const int m = e % TM, k = e / TM;
+          const int gm = m0 + m, gk = kk0 + k;
+          C v;
+          v.x = 0; v.y = 0;
+          if (gm < M && gk < G.K) v = A[gm + static_cast<int64_t>(gk) * G.csA];
+          ra[p] = v;
+        }
       }
       if (G.flags & kSegBNContig) {
 #pragma unroll
@@ -129,10 +146,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
     auto store = [&](const GemmSeg& G) {
       const bool cA = G.flags & kSegConjA, cB = G.flags & kSegConjB, al = G.flags & kSegAlpha;
       const R are = static_cast<R>(G.alpha_re), aim = static_cast<R>(G.alpha_im);
+      const bool akc = G.flags & kSegAKContig;
 #pragma unroll
       for (int p = 0; p < PA; ++p) {
         const int e = tid + p * 256;
-        const int m = e % TM, k = e / TM;
+        const int m = akc ? e / KC : e % TM, k = akc ? e % KC : e / TM;
         R xr = ra[p].x, xi = cA ? -ra[p].y : ra[p].y;
         if (al) {
           const R tr = xr * are - xi * aim;
@@ -218,7 +236,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
         if (n >= N) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int m = m0 + wm * (TM / 2) + i * 16 + 4 * fk + v;
+          const int m = m0 + wm * (TM / 2) + i * 16 + Mf<R>::row(fk, v);
           if (m < M) {
             C* dst = Cp + m + static_cast<int64_t>(n) * P.csC;
             C val;

@@ -11,18 +11,12 @@
 #include "heff.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace cam {
 
 namespace {
-
-struct EnvMat {            // strided view of L_b or R_b for one sector pair
-  int sout;                // output sector (a0' or a2')
-  int64_t off;             // element offset in the tensor data
-  int64_t ld;              // m_out * nb   (column stride over the inner index)
-  int64_t mout, min;
-};
 
 // (b value, input sector) -> view; input sector = s0 (L) or s3 (R).
 std::map<std::pair<int64_t, int>, EnvMat> env_views(const BSTensor& T, int iout, int ib, int iin,
@@ -49,11 +43,6 @@ std::map<std::pair<int64_t, int>, EnvMat> env_views(const BSTensor& T, int iout,
   }
   return out;
 }
-
-struct WMat {             // W(s', bl, s, br) for fixed (bl, br) and sectors
-  int64_t off, ld;        // W^T(k=s, n=s') at off + n + k*ld
-  int64_t nout, nin;
-};
 
 }  // namespace
 
@@ -109,11 +98,11 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
   const bool chkL = L->total <= kHostCheckLimit, chkR = R->total <= kHostCheckLimit;
   if (chkL) hL = download(*L, s);
   if (chkR) hR = download(*R, s);
-  auto Lv = env_views(*L, 1, 3, 5, hL, chkL);   // (b, s0) -> a0' view
-  auto Rv = env_views(*R, 0, 2, 4, hR, chkR);   // (b, s3) -> a2' view
+  Lv = env_views(*L, 1, 3, 5, hL, chkL);   // (b, s0) -> a0' view
+  Rv = env_views(*R, 0, 2, 4, hR, chkR);   // (b, s3) -> a2' view
   std::vector<cplx> hW = download(*W, s);
   // W views: key (bl, br, s', s)
-  std::map<std::array<int64_t, 4>, WMat> Wv;
+  Wv.clear();
   for (const Block& bl : W->blocks) {
     const int sp = bl.sec[0], sbl = bl.sec[1], ss = bl.sec[2], sbr = bl.sec[3];
     const int64_t no = bl.dims[0], nbl = bl.dims[1], ni = bl.dims[2], nbr = bl.dims[3];
@@ -333,6 +322,111 @@ void Heff::apply(const cplx* in, cplx* out, int precision) {
   launch_to_fp32(in, vinf.get(), n, s);
   apply_blocked32(vinf.get(), voutf.get());
   launch_to_fp64(voutf.get(), out, n, s);
+}
+
+TensorPtr Heff::tensor_from_vector(const cplx* x) {
+  auto t = std::make_shared<BSTensor>();
+  t->rules = rules;
+  QIndex s1 = sig, s2 = sig, i0 = a0, i2 = a2;
+  s1.inward = s2.inward = true;
+  i0.inward = true;
+  i2.inward = false;
+  t->idx = {s1, i0, s2, i2};
+  t->numrows = 2;
+  for (const PB& P : pblocks) {
+    Block b;
+    b.sec = {P.s1, P.s0, P.s2, P.s3};
+    b.dims = {P.n1, P.m0, P.n2, P.m3};
+    t->blocks.push_back(b);
+  }
+  std::sort(t->blocks.begin(), t->blocks.end(),
+            [&](const Block& a, const Block& b) { return t->code(a.sec) < t->code(b.sec); });
+  t->finalize_layout();
+  std::vector<PermJob> jobs;
+  for (const PB& P : pblocks) {
+    const Block& o = t->blocks[t->find_block({P.s1, P.s0, P.s2, P.s3})];
+    PermJob j{};
+    j.in = x + P.off;
+    j.out = t->data.get() + o.offset;
+    j.size = P.size;
+    j.ndim = 4;
+    // output (s1, a0, s2, a2) <- block layout (a0, a2, s1, s2)
+    j.odims[0] = P.n1; j.istr[0] = P.m0 * P.m3;
+    j.odims[1] = P.m0; j.istr[1] = 1;
+    j.odims[2] = P.n2; j.istr[2] = P.m0 * P.m3 * P.n1;
+    j.odims[3] = P.m3; j.istr[3] = P.m0;
+    if (j.size) jobs.push_back(j);
+  }
+  launch_permute(jobs, ctx->perm_jobs, ctx->perm_chunks, ctx->stream);
+  return t;
+}
+
+void Heff::vector_from_tensor(const BSTensor& T, cplx* x) {
+  CAM_REQUIRE(T.nidx() == 4, "wavefunction must have 4 indices");
+  CAM_REQUIRE(T.idx[0].same_charges(sig) && T.idx[2].same_charges(sig) && T.idx[1].same_charges(a0) &&
+                  T.idx[3].same_charges(a2),
+              "wavefunction indices do not match the effective Hamiltonian");
+  launch_zero(x, n * sizeof(cplx), ctx->stream);
+  std::vector<PermJob> jobs;
+  for (const Block& b : T.blocks) {
+    auto it = pindex.find({b.sec[1], b.sec[3], b.sec[0], b.sec[2]});
+    CAM_REQUIRE(it != pindex.end(), "wavefunction block outside the allowed space");
+    const PB& P = pblocks[it->second];
+    PermJob j{};
+    j.in = T.data.get() + b.offset;
+    j.out = x + P.off;
+    j.size = P.size;
+    j.ndim = 4;
+    // output (a0, a2, s1, s2) <- tensor layout (s1, a0, s2, a2)
+    j.odims[0] = P.m0; j.istr[0] = P.n1;
+    j.odims[1] = P.m3; j.istr[1] = P.n1 * P.m0 * P.n2;
+    j.odims[2] = P.n1; j.istr[2] = 1;
+    j.odims[3] = P.n2; j.istr[3] = P.n1 * P.m0;
+    if (j.size) jobs.push_back(j);
+  }
+  launch_permute(jobs, ctx->perm_jobs, ctx->perm_chunks, ctx->stream);
+}
+
+void Heff::profile(int precision, int reps, double* stage_ms) {
+  hipStream_t s = ctx->stream;
+  if (precision == 1) prepare_fp32();
+  DevBuf<cplx> x(std::max<int64_t>(n, 1)), y(std::max<int64_t>(n, 1));
+  DevBuf<float2> xf, yf;
+  // deterministic non-zero input
+  std::vector<cplx> h(static_cast<size_t>(std::min<int64_t>(n, 1 << 20)));
+  for (size_t i = 0; i < h.size(); ++i) h[i] = cplx{std::sin(0.37 * i), std::cos(0.11 * i)};
+  for (int64_t o = 0; o < n; o += static_cast<int64_t>(h.size()))
+    HIP_CHECK(hipMemcpyAsync(x.get() + o, h.data(), std::min<int64_t>(h.size(), n - o) * sizeof(cplx),
+                             hipMemcpyHostToDevice, s));
+  const void* in = x.get();
+  void* out = y.get();
+  if (precision == 1) {
+    xf.alloc(std::max<int64_t>(n, 1));
+    yf.alloc(std::max<int64_t>(n, 1));
+    launch_to_fp32(x.get(), xf.get(), n, s);
+    in = xf.get();
+    out = yf.get();
+  }
+  const GemmBases b = bases(precision, in, out);
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+  double acc[4] = {0, 0, 0, 0};
+  for (int r = 0; r < reps + 1; ++r) {            // first pass is a warmup
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    for (int i = 0; i < 4; ++i) {
+      launch_grouped_gemm(st[i], precision, s, &b);
+      HIP_CHECK(hipEventRecord(ev[i + 1], s));
+    }
+    HIP_CHECK(hipEventSynchronize(ev[4]));
+    if (r == 0) continue;
+    for (int i = 0; i < 4; ++i) {
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      acc[i] += ms;
+    }
+  }
+  for (auto& e : ev) HIP_CHECK(hipEventDestroy(e));
+  for (int i = 0; i < 4; ++i) stage_ms[i] = reps > 0 ? acc[i] / reps : 0.0;
 }
 
 void Heff::build_perm() {

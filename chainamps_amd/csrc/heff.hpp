@@ -22,6 +22,18 @@ namespace cam {
 // operand spaces of the stage plans
 enum HeffSpace { kSpL = 1, kSpW, kSpR, kSpT1, kSpT2, kSpT3, kSpIn, kSpOut };
 
+struct EnvMat {            // strided view of L_b or R_b for one sector pair
+  int sout;                // output sector (a0' or a2')
+  int64_t off;             // element offset in the tensor data
+  int64_t ld;              // m_out * nb   (column stride over the inner index)
+  int64_t mout, min;
+};
+
+struct WMat {              // W(s', bl, s, br) for fixed (bl, br) and sectors
+  int64_t off, ld;         // W(s', s) at off + s' + s*ld
+  int64_t nout, nin;
+};
+
 struct Heff {
   Ctx* ctx = nullptr;
   TensorPtr L, W, R;       // keep the inputs alive (shared)
@@ -34,6 +46,10 @@ struct Heff {
   std::map<std::array<int, 4>, int> pindex;   // (s0,s3,s1,s2) -> block
   int64_t n = 0;
 
+  // operand views: L (b, s0) -> a0' view; R (b, s3) -> a2' view;
+  // W (bl, br, s', s) -> view
+  std::map<std::pair<int64_t, int>, EnvMat> Lv, Rv;
+  std::map<std::array<int64_t, 4>, WMat> Wv;
   // intermediates: key (b, s0', s3, sx, sy) -> offset; sizes
   struct TB { int64_t off, m0, m3, n1, n2; };
   std::map<std::array<int, 5>, TB> t1, t2, t3;
@@ -61,6 +77,13 @@ struct Heff {
   void apply_blocked32(const float2* in, float2* out);
   // precision: 0 fp64, 1 fp32 (input/output FP64 block layout)
   void apply(const cplx* in, cplx* out, int precision);
+  // psi as the 4-index wavefunction [s1(in), a0(in), s2(in), a2(out)] with
+  // two row indices (TwoVertexWavefunction, DMRG_routines.cpp:765-766)
+  TensorPtr tensor_from_vector(const cplx* x);
+  // block-layout vector from such a tensor; blocks it lacks are zero
+  void vector_from_tensor(const BSTensor& T, cplx* x);
+  // average ms per stage over `reps` matvecs (HIP events on the stream)
+  void profile(int precision, int reps, double* stage_ms);
 };
 
 }  // namespace cam

@@ -1,0 +1,352 @@
+// Device Lanczos for the lowest eigenpair of H_eff.
+//
+// Replaces arpack_eigs<TwoVertexComponents,SparseVectorWithRestriction>
+// (arpack_interface.hpp:85-111, 174-237) as called from
+// TwoVertexComponents::HED (DMRG_routines.cpp:835-896): nev=1, which="SR",
+// Krylov dimension ncv (reference 20, arpack_interface.cpp:36-37),
+// convergence when the Ritz residual |beta_m s_m| <= tol*|theta| (ARPACK's
+// criterion), tol=0.1 for the non-converged growth steps
+// (arpack_interface.cpp:28-31), machine epsilon when converging.  Restarts
+// use the current Ritz vector (ARPACK restarts implicitly; the converged
+// pair is the same).
+//
+// All vectors stay on the device in the block layout; Krylov vectors are
+// kept fully orthogonal with two passes of classical Gram-Schmidt in FP64;
+// the tridiagonal eigenproblem is solved in FP64 on the device with
+// rocSOLVER dsteqr.
+//
+// Mixed precision: each restart starts from the FP64 Ritz vector x, whose
+// FP64 matvec gives the exact Rayleigh quotient and residual r; the Krylov
+// space {x, r, H r, ...} is then extended with FP32 split-complex matvecs.
+// FP32 errors enter only the correction, so the FP64 residual keeps
+// decreasing; when it stalls, or once it is within 1e3*eps*|theta|, the
+// remaining restarts run in FP64.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <limits>
+
+#include "solvers.hpp"
+
+namespace cam {
+
+void* Ctx::blas_handle() {
+  if (!blas) {
+    rocblas_handle h;
+    CAM_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
+    CAM_REQUIRE(rocblas_set_stream(h, stream) == rocblas_status_success, "rocblas_set_stream failed");
+    blas = h;
+  }
+  return blas;
+}
+
+Ctx::~Ctx() {
+  if (blas) rocblas_destroy_handle(static_cast<rocblas_handle>(blas));
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+__global__ void random_fill_kernel(double2* x, int64_t n, uint64_t seed) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double a = (z >> 11) * (1.0 / 9007199254740992.0);
+    z = z * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z ^= z >> 29;
+    const double b = (z >> 11) * (1.0 / 9007199254740992.0);
+    x[i] = make_double2(2.0 * a - 1.0, 2.0 * b - 1.0);
+  }
+}
+
+struct Lanczos {
+  Heff& H;
+  hipStream_t s;
+  int64_t n;
+  int m;                      // Krylov dimension
+  DevBuf<cplx> V;             // (m+1) x n
+  DevBuf<cplx> w, x, y;       // work
+  DevBuf<cplx> coef, partial;
+  DevBuf<double> dD, dE, dC;
+  DevBuf<int> dinfo;
+  int mv64 = 0, mv32 = 0;
+
+  Lanczos(Heff& h, int ncv) : H(h), s(h.ctx->stream), n(h.n) {
+    m = std::max(1, static_cast<int>(std::min<int64_t>(ncv, n)));
+    CAM_REQUIRE(m + 1 <= 64, "ncv too large (max 63)");
+    V.alloc(static_cast<size_t>(n) * (m + 1));
+    w.alloc(n);
+    x.alloc(n);
+    y.alloc(n);
+    coef.alloc(64);
+    partial.alloc(static_cast<size_t>(multi_dot_blocks(n)) * 64);
+    dD.alloc(64);
+    dE.alloc(64);
+    dC.alloc(64 * 64);
+    dinfo.alloc(1);
+  }
+  cplx* col(int j) { return V.get() + static_cast<int64_t>(j) * n; }
+
+  void matvec(const cplx* in, cplx* out, int prec) {
+    H.apply(in, out, prec);
+    if (prec == 0) ++mv64; else ++mv32;
+  }
+  std::vector<std::complex<double>> dots(int k, const cplx* vec) {
+    launch_multi_dot(V.get(), n, k, vec, n, partial.get(), coef.get(), s);
+    std::vector<std::complex<double>> c(k);
+    HIP_CHECK(hipMemcpyAsync(c.data(), coef.get(), k * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return c;
+  }
+  double norm(const cplx* vec) {
+    // <vec,vec> through the multi-dot with V := vec
+    launch_multi_dot(vec, n, 1, vec, n, partial.get(), coef.get(), s);
+    std::complex<double> c;
+    HIP_CHECK(hipMemcpyAsync(&c, coef.get(), sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return std::sqrt(std::max(0.0, c.real()));
+  }
+  // w -= V[:, :k] (V^H w), twice; returns the first-pass coefficients + corrections
+  std::vector<std::complex<double>> orthogonalize(int k, cplx* vec) {
+    std::vector<std::complex<double>> tot(k, 0.0);
+    for (int pass = 0; pass < 2; ++pass) {
+      auto c = dots(k, vec);
+      for (int j = 0; j < k; ++j) tot[j] += c[j];
+      HIP_CHECK(hipMemcpyAsync(coef.get(), c.data(), k * sizeof(cplx), hipMemcpyHostToDevice, s));
+      launch_multi_axpy(V.get(), n, k, coef.get(), vec, n, s);
+    }
+    return tot;
+  }
+  // lowest eigenpair of the symmetric tridiagonal (alpha, beta) on the device
+  double tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, std::vector<double>& vec) {
+    const int k = static_cast<int>(a.size());
+    rocblas_handle h = static_cast<rocblas_handle>(H.ctx->blas_handle());
+    std::vector<double> e(std::max(k - 1, 1), 0.0);
+    for (int i = 0; i + 1 < k; ++i) e[i] = b[i];
+    HIP_CHECK(hipMemcpyAsync(dD.get(), a.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dE.get(), e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    const rocblas_status st =
+        rocsolver_dsteqr(h, rocblas_evect_tridiagonal, k, dD.get(), dE.get(), dC.get(), k, dinfo.get());
+    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_dsteqr failed");
+    std::vector<double> evals(k), C(static_cast<size_t>(k) * k);
+    int info = 0;
+    HIP_CHECK(hipMemcpyAsync(evals.data(), dD.get(), k * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(C.data(), dC.get(), C.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    CAM_REQUIRE(info == 0, "tridiagonal eigensolver did not converge");
+    int lo = 0;
+    for (int i = 1; i < k; ++i)
+      if (evals[i] < evals[lo]) lo = i;
+    vec.assign(C.begin() + static_cast<int64_t>(lo) * k, C.begin() + static_cast<int64_t>(lo + 1) * k);
+    return evals[lo];
+  }
+};
+
+}  // namespace
+
+namespace {
+
+__global__ void absmax_partial_kernel(const double2* __restrict__ x, int64_t n, double* pv, int64_t* pi) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  double best = -1.0;
+  int64_t bi = 0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = x[i];
+    const double a = v.x * v.x + v.y * v.y;
+    if (a > best || (a == best && i < bi)) { best = a; bi = i; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double a = sv[threadIdx.x + o];
+      const int64_t b = si[threadIdx.x + o];
+      if (a > sv[threadIdx.x] || (a == sv[threadIdx.x] && b < si[threadIdx.x])) {
+        sv[threadIdx.x] = a;
+        si[threadIdx.x] = b;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { pv[blockIdx.x] = sv[0]; pi[blockIdx.x] = si[0]; }
+}
+
+// x <- FixComplexPrecision(x * phase) with |x*phase| <= thr set to zero
+__global__ void gauge_kernel(double2* __restrict__ x, int64_t n, double pr, double pim, double thr,
+                             double eps) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = x[i];
+    double re = v.x * pr - v.y * pim, im = v.x * pim + v.y * pr;
+    if (re * re + im * im > thr * thr) {
+      if (re != 0.0 && fabs(im / re) < eps) im = 0.0;
+      else if (im != 0.0 && fabs(re / im) < eps) re = 0.0;
+    } else {
+      re = 0.0;
+      im = 0.0;
+    }
+    x[i] = make_double2(re, im);
+  }
+}
+
+}  // namespace
+
+// Phase gauge of DMRG_routines.cpp:870-890 on a device vector: the
+// largest-modulus entry made real positive, entries below eps*max dropped,
+// FixComplexPrecision (sparse_interface.cpp:28-32) on the rest.  In block
+// layout the "largest entry" is the same element as in reference order;
+// ties are broken by block-layout position.
+void gauge_fix(cplx* x, int64_t n, hipStream_t s) {
+  const int nb = 256;
+  DevBuf<double> pv(nb);
+  DevBuf<int64_t> pi(nb);
+  hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(256), 0, s, reinterpret_cast<const double2*>(x), n,
+                     pv.get(), pi.get());
+  HIP_CHECK(hipGetLastError());
+  std::vector<double> hv(nb);
+  std::vector<int64_t> hi(nb);
+  HIP_CHECK(hipMemcpyAsync(hv.data(), pv.get(), nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hi.data(), pi.get(), nb * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  int b = 0;
+  for (int k = 1; k < nb; ++k)
+    if (hv[k] > hv[b] || (hv[k] == hv[b] && hi[k] < hi[b])) b = k;
+  cplx mv;
+  HIP_CHECK(hipMemcpy(&mv, x + hi[b], sizeof(cplx), hipMemcpyDeviceToHost));
+  const double mx = std::sqrt(mv.re * mv.re + mv.im * mv.im);
+  if (mx == 0.0) return;
+  // phase = mx / mv
+  const double pr = mv.re / mx, pim = -mv.im / mx;
+  const double eps = std::numeric_limits<double>::epsilon();
+  hipLaunchKernelGGL(gauge_kernel, dim3(1024), dim3(256), 0, s, reinterpret_cast<double2*>(x), n, pr, pim,
+                     eps * mx, eps);
+  HIP_CHECK(hipGetLastError());
+}
+
+double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, double tol, int ncv,
+                      bool mixed, double* info) {
+  const int64_t n = H.n;
+  CAM_REQUIRE(n > 0, "empty eigenproblem");
+  hipStream_t s = H.ctx->stream;
+  const double eps = std::numeric_limits<double>::epsilon();
+  double tol_eff = converge ? (tol > 0 ? tol : eps) : (tol > 0 ? tol : 0.1);
+  // the residual of an FP64 matvec cannot go far below ~eps*|H|; the
+  // converged criterion is taken as max(tol, 64 eps)
+  tol_eff = std::max(tol_eff, 64 * eps);
+  ncv = ncv > 0 ? ncv : 20;
+  Lanczos L(H, ncv);
+  // start vector (SparseVectorWithRestriction -> DumbExtractWithZeros); a
+  // zero guess takes the info=-9 path: random start
+  if (have_start) {
+    launch_copy(xout, L.x.get(), n, s);
+    have_start = L.norm(L.x.get()) > 0.0;
+  }
+  if (!have_start) {
+    hipLaunchKernelGGL(random_fill_kernel, dim3(1024), dim3(256), 0, s,
+                       reinterpret_cast<double2*>(L.x.get()), n, 0x1234ull);
+    HIP_CHECK(hipGetLastError());
+  }
+  {
+    const double nx = L.norm(L.x.get());
+    launch_scale(cplx{1.0 / nx, 0.0}, L.x.get(), n, s);
+  }
+  const int max_restarts = converge ? 200 : 100;
+  double theta = 0.0, resid = 0.0, prev_resid = 1e300;
+  int restarts = 0;
+  bool use32 = mixed;
+  while (true) {
+    const int m = L.m;
+    launch_copy(L.x.get(), L.col(0), n, s);
+    // step 0 in FP64: exact Rayleigh quotient and residual of x
+    L.matvec(L.col(0), L.w.get(), 0);
+    auto c0 = L.orthogonalize(1, L.w.get());
+    theta = c0[0].real();
+    resid = L.norm(L.w.get());
+    if (resid <= tol_eff * std::max(std::fabs(theta), 1e-300) || restarts >= max_restarts) break;
+    if (use32 && restarts > 1 && (resid > 0.5 * prev_resid || resid < 1e3 * eps * std::fabs(theta)))
+      use32 = false;   // final FP64 Lanczos steps
+    prev_resid = resid;
+    std::vector<double> alpha{theta}, beta;
+    bool invariant = resid <= 16 * eps * std::max(1.0, std::fabs(theta));
+    int k = 1;
+    if (!invariant) {
+      beta.push_back(resid);
+      launch_scale(cplx{1.0 / resid, 0.0}, L.w.get(), n, s);
+      launch_copy(L.w.get(), L.col(1), n, s);
+      for (int j = 1; j < m; ++j) {
+        L.matvec(L.col(j), L.w.get(), use32 ? 1 : 0);
+        auto c = L.orthogonalize(j + 1, L.w.get());
+        alpha.push_back(c[j].real());
+        const double b = L.norm(L.w.get());
+        k = j + 1;
+        if (b <= 16 * eps * std::max(1.0, std::fabs(alpha.back()))) {
+          invariant = true;
+          break;
+        }
+        beta.push_back(b);
+        if (j + 1 < m) {
+          launch_scale(cplx{1.0 / b, 0.0}, L.w.get(), n, s);
+          launch_copy(L.w.get(), L.col(j + 1), n, s);
+        }
+      }
+    }
+    std::vector<double> svec;
+    beta.resize(k, 0.0);
+    const double th = L.tridiag_lowest(alpha, beta, svec);
+    const double ritz_res = invariant ? 0.0 : std::fabs(beta[k - 1] * svec[k - 1]);
+    std::vector<std::complex<double>> cs(k);
+    for (int j = 0; j < k; ++j) cs[j] = svec[j];
+    HIP_CHECK(hipMemcpyAsync(L.coef.get(), cs.data(), k * sizeof(cplx), hipMemcpyHostToDevice, s));
+    launch_combine(L.V.get(), n, k, L.coef.get(), L.x.get(), n, s);
+    const double nx = L.norm(L.x.get());
+    launch_scale(cplx{1.0 / nx, 0.0}, L.x.get(), n, s);
+    ++restarts;
+    // FP64 runs stop on ARPACK's Ritz estimate; mixed runs re-check the new
+    // vector with an exact FP64 matvec at the top of the loop
+    if (!use32 && (invariant || ritz_res <= tol_eff * std::max(std::fabs(th), 1e-300))) {
+      theta = th;
+      resid = ritz_res;
+      break;
+    }
+  }
+  if (info) {
+    info[0] = L.mv64;
+    info[1] = L.mv32;
+    info[2] = restarts;
+    info[3] = resid;
+  }
+  launch_copy(L.x.get(), xout, n, s);
+  gauge_fix(xout, n, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  return theta;
+}
+
+void heff_lowest(Heff& H, const cplx* guess_ref, bool converge, double tol, int ncv, bool mixed,
+                 double* eval, cplx* evec_ref, double* info) {
+  const int64_t n = H.n;
+  hipStream_t s = H.ctx->stream;
+  H.build_perm();
+  DevBuf<cplx> x(n), y(n);
+  if (guess_ref) {
+    y.upload(guess_ref, n, s);
+    launch_scatter_map(y.get(), x.get(), H.perm.get(), n, s);
+  }
+  *eval = lanczos_lowest(H, x.get(), guess_ref != nullptr, converge, tol, ncv, mixed, info);
+  if (evec_ref) {
+    launch_gather_map(x.get(), y.get(), H.perm.get(), n, s);
+    HIP_CHECK(hipMemcpyAsync(evec_ref, y.get(), n * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+}  // namespace cam

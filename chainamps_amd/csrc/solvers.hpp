@@ -11,6 +11,14 @@ namespace cam {
 void heff_lowest(Heff& H, const cplx* guess_ref, bool converge, double tol, int ncv, bool mixed,
                  double* eval, cplx* evec_ref, double* info);
 
+// Block-layout core: x (device, n) holds the start vector when have_start
+// and receives the gauge-fixed eigenvector; returns the eigenvalue.
+double lanczos_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv,
+                      bool mixed, double* info);
+void gauge_fix(cplx* x, int64_t n, hipStream_t s);
+// contract with a diagonal on index k (values per position of that index)
+void scale_along_index(Ctx& ctx, BSTensor& T, int k, const std::vector<double>& lam);
+
 // MPX_matrix::SVD with the reference truncation rule.
 void svd_truncate(Ctx& ctx, const BSTensor& theta, int64_t D, double max_trunc, TensorPtr& U,
                   TensorPtr& Vd, std::vector<double>& values, double& discarded);

@@ -1,0 +1,446 @@
+// Blockwise SVD with the reference truncation rule, FP64 on the device.
+//
+// MPX_matrix::SVD (MPX_matrix.cpp:157-192) -> SparseMatrix::SVD
+// (sparse_interface.cpp:1248-1406):
+//   * columns are grouped by OutgoingColState (GetAllBlockColumns,
+//     MPX_matrix.cpp:331-355); in block storage a group is the set of
+//     blocks whose column sectors carry that charge;
+//   * a group whose squared weight is <= eps^2 is skipped;
+//   * the dense block keeps only non-empty rows and columns
+//     (TranslationBlock, sparse_interface.hpp:353-371) and is decomposed
+//     by economy SVD (the reference calls zgesdd with JOB='S',
+//     dense_matrix_functions.cpp:276-337; here rocSOLVER zgesvd);
+//   * all singular values are sorted descending (ties: block order, then
+//     index inside the block -- std::sort leaves ties unspecified), the
+//     kept length is min(D, #values) (D=0: all), cut further while the
+//     discarded squared tail stays below max_trunc, then extended through
+//     relative gaps < 1e-3;
+//   * U entries with |x| <= eps/rows and V^dagger entries <= eps/cols are
+//     dropped; the new bond carries the ingoing row state of each U column
+//     (MPX_matrix.cpp:176-180), i.e. the group charge.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <numeric>
+
+#include "solvers.hpp"
+
+namespace cam {
+
+namespace {
+
+struct FlagJob {
+  const double2* in;
+  int64_t M, N;          // block viewed as (rows x cols) column-major
+  int* rflag;
+  int* cflag;
+};
+
+__global__ void nonzero_flags_kernel(const FlagJob* jobs, int njobs) {
+  for (int j = blockIdx.y; j < njobs; j += gridDim.y) {
+    const FlagJob J = jobs[j];
+    const int64_t tot = J.M * J.N;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot;
+         e += int64_t(gridDim.x) * blockDim.x) {
+      const double2 v = J.in[e];
+      if (v.x != 0.0 || v.y != 0.0) {
+        J.rflag[e % J.M] = 1;
+        J.cflag[e / J.M] = 1;
+      }
+    }
+  }
+}
+
+// out[i + j*ldo] = (r[i] >= 0 && c[j] >= 0) ? in[r[i]*rs + c[j]*cs] : 0,
+// with |x| <= thr dropped to 0
+struct GatherJob {
+  const double2* in;
+  double2* out;
+  const int64_t* r;
+  const int64_t* c;
+  int64_t M, N, ldo, rs, cs;
+  double thr;
+};
+
+__global__ void gather2d_kernel(const GatherJob* jobs, int njobs) {
+  for (int j = blockIdx.y; j < njobs; j += gridDim.y) {
+    const GatherJob J = jobs[j];
+    const int64_t tot = J.M * J.N;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot;
+         e += int64_t(gridDim.x) * blockDim.x) {
+      const int64_t i = e % J.M, jj = e / J.M;
+      const int64_t ri = J.r[i], cj = J.c[jj];
+      double2 v = make_double2(0.0, 0.0);
+      if (ri >= 0 && cj >= 0) {
+        v = J.in[ri * J.rs + cj * J.cs];
+        if (v.x * v.x + v.y * v.y <= J.thr * J.thr) v = make_double2(0.0, 0.0);
+      }
+      J.out[i + jj * J.ldo] = v;
+    }
+  }
+}
+
+template <typename J, typename K>
+void run_jobs(K kernel, const std::vector<J>& jobs, hipStream_t s) {
+  if (jobs.empty()) return;
+  DevBuf<J> d(jobs.size());
+  d.upload(jobs.data(), jobs.size(), s);
+  const int gy = static_cast<int>(std::min<size_t>(jobs.size(), 1024));
+  hipLaunchKernelGGL(kernel, dim3(64, gy), dim3(256), 0, s, d.get(), static_cast<int>(jobs.size()));
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+struct Tuple {
+  std::vector<int32_t> sec;
+  std::vector<int64_t> dims;
+  int64_t size = 1;
+  bool operator<(const Tuple& o) const { return sec < o.sec; }
+};
+
+struct Group {
+  Charge q;
+  std::vector<Tuple> rows, cols;                 // distinct sector tuples
+  std::map<std::vector<int32_t>, int> rpos, cpos;
+  std::vector<int64_t> roff, coff;               // uncompacted offsets of tuples
+  int64_t rtot = 0, ctot = 0;
+  std::vector<int> blocks;
+  // compaction
+  std::vector<int64_t> rmap, cmap;               // uncompacted -> compact (-1 empty)
+  int64_t mg = 0, ng = 0;
+  int64_t first_col = 0;                         // GetAllBlockColumns order key
+  double weight = 0.0;
+  int64_t k = 0;                                 // number of singular values
+  DevBuf<cplx> A, U, Vh;
+  std::vector<double> S;
+};
+
+}  // namespace
+
+void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorPtr& Uout,
+                  TensorPtr& Vdout, std::vector<double>& values, double& discarded) {
+  hipStream_t s = ctx.stream;
+  const double eps = std::numeric_limits<double>::epsilon();
+  const int r = th.numrows, N = th.nidx();
+  CAM_REQUIRE(r >= 1 && r < N, "SVD needs at least one row and one column index");
+  // ---- groups of blocks by column state -----------------------------------
+  std::map<Charge, Group> groups;
+  for (size_t b = 0; b < th.blocks.size(); ++b) {
+    const Block& bl = th.blocks[b];
+    Charge qc = th.rules.zero();
+    for (int k = r; k < N; ++k) {
+      const Charge& q = th.idx[k].sec_q[bl.sec[k]];
+      qc = th.idx[k].inward ? th.rules.sub(qc, q) : th.rules.add(qc, q);
+    }
+    Group& g = groups[qc];
+    g.q = qc;
+    g.blocks.push_back(static_cast<int>(b));
+    Tuple rt, ct;
+    rt.sec.assign(bl.sec.begin(), bl.sec.begin() + r);
+    rt.dims.assign(bl.dims.begin(), bl.dims.begin() + r);
+    ct.sec.assign(bl.sec.begin() + r, bl.sec.end());
+    ct.dims.assign(bl.dims.begin() + r, bl.dims.end());
+    for (auto d : rt.dims) rt.size *= d;
+    for (auto d : ct.dims) ct.size *= d;
+    if (!g.rpos.count(rt.sec)) { g.rpos[rt.sec] = -1; g.rows.push_back(rt); }
+    if (!g.cpos.count(ct.sec)) { g.cpos[ct.sec] = -1; g.cols.push_back(ct); }
+  }
+  // column strides of the flat column index (for the group order)
+  std::vector<int64_t> cstride(N, 0);
+  {
+    int64_t m = 1;
+    for (int k = r; k < N; ++k) { cstride[k] = m; m *= th.idx[k].size; }
+  }
+  std::vector<Group*> glist;
+  for (auto& kv : groups) {
+    Group& g = kv.second;
+    std::sort(g.rows.begin(), g.rows.end());
+    std::sort(g.cols.begin(), g.cols.end());
+    for (size_t i = 0; i < g.rows.size(); ++i) {
+      g.rpos[g.rows[i].sec] = static_cast<int>(i);
+      g.roff.push_back(g.rtot);
+      g.rtot += g.rows[i].size;
+    }
+    g.first_col = std::numeric_limits<int64_t>::max();
+    for (size_t i = 0; i < g.cols.size(); ++i) {
+      g.cpos[g.cols[i].sec] = static_cast<int>(i);
+      g.coff.push_back(g.ctot);
+      g.ctot += g.cols[i].size;
+      int64_t fc = 0;
+      for (int k = r; k < N; ++k) fc += th.idx[k].sec_pos[g.cols[i].sec[k - r]][0] * cstride[k];
+      g.first_col = std::min(g.first_col, fc);
+    }
+    glist.push_back(&g);
+  }
+  std::sort(glist.begin(), glist.end(), [](const Group* a, const Group* b) { return a->first_col < b->first_col; });
+
+  // ---- non-empty rows / columns ---------------------------------------------
+  int64_t ftot = 0;
+  std::vector<int64_t> fbase(glist.size());
+  for (size_t gi = 0; gi < glist.size(); ++gi) {
+    fbase[gi] = ftot;
+    ftot += glist[gi]->rtot + glist[gi]->ctot;
+  }
+  DevBuf<int> dflags(std::max<int64_t>(ftot, 1));
+  launch_zero(dflags.get(), std::max<int64_t>(ftot, 1) * sizeof(int), s);
+  {
+    std::vector<FlagJob> jobs;
+    for (size_t gi = 0; gi < glist.size(); ++gi) {
+      Group& g = *glist[gi];
+      for (int b : g.blocks) {
+        const Block& bl = th.blocks[b];
+        std::vector<int32_t> rs(bl.sec.begin(), bl.sec.begin() + r), cs(bl.sec.begin() + r, bl.sec.end());
+        const int ri = g.rpos[rs], ci = g.cpos[cs];
+        FlagJob j{reinterpret_cast<const double2*>(th.data.get() + bl.offset), g.rows[ri].size,
+                  g.cols[ci].size, dflags.get() + fbase[gi] + g.roff[ri],
+                  dflags.get() + fbase[gi] + g.rtot + g.coff[ci]};
+        if (j.M * j.N) jobs.push_back(j);
+      }
+    }
+    run_jobs(nonzero_flags_kernel, jobs, s);
+  }
+  std::vector<int> flags(std::max<int64_t>(ftot, 1));
+  HIP_CHECK(hipMemcpy(flags.data(), dflags.get(), std::max<int64_t>(ftot, 1) * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t gi = 0; gi < glist.size(); ++gi) {
+    Group& g = *glist[gi];
+    g.rmap.assign(g.rtot, -1);
+    g.cmap.assign(g.ctot, -1);
+    for (int64_t i = 0; i < g.rtot; ++i)
+      if (flags[fbase[gi] + i]) g.rmap[i] = g.mg++;
+    for (int64_t j = 0; j < g.ctot; ++j)
+      if (flags[fbase[gi] + g.rtot + j]) g.cmap[j] = g.ng++;
+  }
+
+  // ---- assemble dense group matrices and decompose -----------------------
+  rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
+  DevBuf<int> dinfo(1);
+  DevBuf<double> dE, dS;
+  DevBuf<cplx> partial(64 * 1024), dcoef(64);
+  for (size_t gi = 0; gi < glist.size(); ++gi) {
+    Group& g = *glist[gi];
+    if (g.mg == 0 || g.ng == 0) continue;
+    CAM_REQUIRE(g.mg < (int64_t(1) << 31) && g.ng < (int64_t(1) << 31), "SVD block too large");
+    g.A.alloc(g.mg * g.ng);
+    launch_zero(g.A.get(), g.mg * g.ng * sizeof(cplx), s);
+    // scatter the blocks: gather2d with the output as the compact matrix
+    std::vector<GatherJob> jobs;
+    std::vector<std::vector<int64_t>> keep;
+    for (int b : g.blocks) {
+      const Block& bl = th.blocks[b];
+      std::vector<int32_t> rs(bl.sec.begin(), bl.sec.begin() + r), cs(bl.sec.begin() + r, bl.sec.end());
+      const int ri = g.rpos[rs], ci = g.cpos[cs];
+      // iterate over compact rows/cols that this block covers
+      std::vector<int64_t> rr, cc, orow, ocol;
+      for (int64_t i = 0; i < g.rows[ri].size; ++i)
+        if (g.rmap[g.roff[ri] + i] >= 0) { rr.push_back(i); orow.push_back(g.rmap[g.roff[ri] + i]); }
+      for (int64_t j = 0; j < g.cols[ci].size; ++j)
+        if (g.cmap[g.coff[ci] + j] >= 0) { cc.push_back(j); ocol.push_back(g.cmap[g.coff[ci] + j]); }
+      if (rr.empty() || cc.empty()) continue;
+      // compact rows/cols of one block are contiguous ranges in the group
+      // matrix (tuples are laid out back to back): out = A + orow0 + ocol0*mg
+      keep.push_back(rr);
+      keep.push_back(cc);
+      GatherJob j{};
+      j.in = reinterpret_cast<const double2*>(th.data.get() + bl.offset);
+      j.out = reinterpret_cast<double2*>(g.A.get() + orow.front() + ocol.front() * g.mg);
+      j.M = static_cast<int64_t>(rr.size());
+      j.N = static_cast<int64_t>(cc.size());
+      j.ldo = g.mg;
+      j.rs = 1;
+      j.cs = g.rows[ri].size;
+      j.thr = 0.0;
+      jobs.push_back(j);
+    }
+    // upload the row/col maps
+    std::vector<int64_t> flat;
+    std::vector<size_t> at;
+    for (auto& v : keep) { at.push_back(flat.size()); flat.insert(flat.end(), v.begin(), v.end()); }
+    DevBuf<int64_t> dmaps(std::max<size_t>(flat.size(), 1));
+    dmaps.upload(flat.data(), flat.size(), s);
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      jobs[q].r = dmaps.get() + at[2 * q];
+      jobs[q].c = dmaps.get() + at[2 * q + 1];
+    }
+    run_jobs(gather2d_kernel, jobs, s);
+    // group weight (sum of squared moduli)
+    launch_multi_dot(g.A.get(), g.mg * g.ng, 1, g.A.get(), g.mg * g.ng, partial.get(), dcoef.get(), s);
+    cplx wsum;
+    HIP_CHECK(hipMemcpyAsync(&wsum, dcoef.get(), sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    g.weight = wsum.re;
+    if (g.weight <= eps * eps) continue;                  // empty block: skipped
+    g.k = std::min(g.mg, g.ng);
+    g.U.alloc(g.mg * g.k);
+    g.Vh.alloc(g.k * g.ng);
+    dS.ensure(g.k);
+    dE.ensure(std::max<int64_t>(g.k, 1));
+    const rocblas_status st = rocsolver_zgesvd(
+        h, rocblas_svect_singular, rocblas_svect_singular, static_cast<rocblas_int>(g.mg),
+        static_cast<rocblas_int>(g.ng), reinterpret_cast<rocblas_double_complex*>(g.A.get()),
+        static_cast<rocblas_int>(g.mg), dS.get(), reinterpret_cast<rocblas_double_complex*>(g.U.get()),
+        static_cast<rocblas_int>(g.mg), reinterpret_cast<rocblas_double_complex*>(g.Vh.get()),
+        static_cast<rocblas_int>(g.k), dE.get(), rocblas_outofplace, dinfo.get());
+    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgesvd failed");
+    g.S.resize(g.k);
+    int info = 0;
+    HIP_CHECK(hipMemcpyAsync(g.S.data(), dS.get(), g.k * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    CAM_REQUIRE(info == 0, "blockwise SVD did not converge");
+    g.A.reset();
+  }
+
+  // ---- global sort and truncation rule --------------------------------------
+  struct SV { double v; int g; int64_t i; };
+  std::vector<SV> all;
+  for (size_t gi = 0; gi < glist.size(); ++gi)
+    for (int64_t i = 0; i < glist[gi]->k; ++i) all.push_back(SV{glist[gi]->S[i], static_cast<int>(gi), i});
+  CAM_REQUIRE(!all.empty(), "Error: No singular values. Empty blocks?");
+  std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) {
+    if (a.v != b.v) return a.v > b.v;
+    if (a.g != b.g) return a.g < b.g;
+    return a.i < b.i;
+  });
+  const size_t nall = all.size();
+  size_t length = (D > 0 && static_cast<size_t>(D) < nall) ? static_cast<size_t>(D) : nall;
+  {
+    double trunc_error = 0.0;
+    size_t tl = nall;
+    while (trunc_error < max_trunc && tl > 0) {
+      trunc_error += all[tl - 1].v * all[tl - 1].v;
+      --tl;
+    }
+    ++tl;
+    if (tl < length) length = tl;
+    while (length < nall && (all[length - 1].v - all[length].v) / all[length - 1].v < 1.0e-3) ++length;
+  }
+  values.clear();
+  for (size_t i = 0; i < length; ++i) values.push_back(all[i].v);
+  discarded = 0.0;
+  for (size_t i = length; i < nall; ++i) discarded += all[i].v * all[i].v;
+
+  // ---- new bond index and the U / V^dagger tensors ------------------------
+  QIndex nb;
+  nb.physical = false;
+  nb.size = static_cast<int64_t>(length);
+  for (size_t i = 0; i < length; ++i) nb.charges.push_back(glist[all[i].g]->q);
+  nb.build_sectors();
+  std::vector<std::vector<int64_t>> kept(glist.size());     // local singular indices, sorted order
+  for (size_t i = 0; i < length; ++i) kept[all[i].g].push_back(all[i].i);
+
+  auto U = std::make_shared<BSTensor>();
+  U->rules = th.rules;
+  for (int k = 0; k < r; ++k) U->idx.push_back(th.idx[k]);
+  QIndex nbo = nb;
+  nbo.inward = false;
+  U->idx.push_back(nbo);
+  U->numrows = r;
+  auto Vd = std::make_shared<BSTensor>();
+  Vd->rules = th.rules;
+  QIndex nbi = nb;
+  nbi.inward = true;
+  Vd->idx.push_back(nbi);
+  for (int k = r; k < N; ++k) Vd->idx.push_back(th.idx[k]);
+  Vd->numrows = 1;
+  struct UB { int g; int tup; };
+  std::vector<UB> ub, vb;
+  for (size_t gi = 0; gi < glist.size(); ++gi) {
+    Group& g = *glist[gi];
+    if (kept[gi].empty()) continue;
+    const int sq = nb.find_sector(g.q);
+    for (size_t t = 0; t < g.rows.size(); ++t) {
+      Block b;
+      b.sec = g.rows[t].sec;
+      b.sec.push_back(sq);
+      b.dims = g.rows[t].dims;
+      b.dims.push_back(static_cast<int64_t>(kept[gi].size()));
+      U->blocks.push_back(b);
+      ub.push_back(UB{static_cast<int>(gi), static_cast<int>(t)});
+    }
+    for (size_t t = 0; t < g.cols.size(); ++t) {
+      Block b;
+      b.sec.push_back(sq);
+      b.sec.insert(b.sec.end(), g.cols[t].sec.begin(), g.cols[t].sec.end());
+      b.dims.push_back(static_cast<int64_t>(kept[gi].size()));
+      b.dims.insert(b.dims.end(), g.cols[t].dims.begin(), g.cols[t].dims.end());
+      Vd->blocks.push_back(b);
+      vb.push_back(UB{static_cast<int>(gi), static_cast<int>(t)});
+    }
+  }
+  // sort blocks by code but keep the job association
+  auto layout = [](BSTensor& T, std::vector<UB>& tag) {
+    std::vector<size_t> ord(T.blocks.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::vector<uint64_t> codes(T.blocks.size());
+    for (size_t i = 0; i < T.blocks.size(); ++i) codes[i] = T.code(T.blocks[i].sec);
+    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return codes[a] < codes[b]; });
+    std::vector<Block> nbk;
+    std::vector<UB> nt;
+    for (auto o : ord) { nbk.push_back(T.blocks[o]); nt.push_back(tag[o]); }
+    T.blocks.swap(nbk);
+    tag.swap(nt);
+    T.finalize_layout();
+  };
+  layout(*U, ub);
+  layout(*Vd, vb);
+  {
+    std::vector<GatherJob> jobs;
+    std::vector<int64_t> flat;
+    std::vector<std::pair<size_t, size_t>> at;
+    for (size_t bi = 0; bi < U->blocks.size(); ++bi) {
+      Group& g = *glist[ub[bi].g];
+      const int t = ub[bi].tup;
+      size_t a = flat.size();
+      for (int64_t i = 0; i < g.rows[t].size; ++i) flat.push_back(g.rmap[g.roff[t] + i]);
+      size_t c = flat.size();
+      for (auto li : kept[ub[bi].g]) flat.push_back(li);
+      at.push_back({a, c});
+      GatherJob j{};
+      j.in = reinterpret_cast<const double2*>(g.U.get());
+      j.out = reinterpret_cast<double2*>(U->data.get() + U->blocks[bi].offset);
+      j.M = g.rows[t].size;
+      j.N = static_cast<int64_t>(kept[ub[bi].g].size());
+      j.ldo = j.M;
+      j.rs = 1;
+      j.cs = g.mg;
+      j.thr = eps / static_cast<double>(g.mg);
+      jobs.push_back(j);
+    }
+    size_t ubase = jobs.size();
+    for (size_t bi = 0; bi < Vd->blocks.size(); ++bi) {
+      Group& g = *glist[vb[bi].g];
+      const int t = vb[bi].tup;
+      size_t a = flat.size();
+      for (auto li : kept[vb[bi].g]) flat.push_back(li);
+      size_t c = flat.size();
+      for (int64_t j = 0; j < g.cols[t].size; ++j) flat.push_back(g.cmap[g.coff[t] + j]);
+      at.push_back({a, c});
+      GatherJob j{};
+      j.in = reinterpret_cast<const double2*>(g.Vh.get());
+      j.out = reinterpret_cast<double2*>(Vd->data.get() + Vd->blocks[bi].offset);
+      j.M = static_cast<int64_t>(kept[vb[bi].g].size());
+      j.N = g.cols[t].size;
+      j.ldo = j.M;
+      j.rs = 1;
+      j.cs = g.k;
+      j.thr = eps / static_cast<double>(g.ng);
+      jobs.push_back(j);
+    }
+    (void)ubase;
+    DevBuf<int64_t> dmaps(std::max<size_t>(flat.size(), 1));
+    dmaps.upload(flat.data(), flat.size(), s);
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      jobs[q].r = dmaps.get() + at[q].first;
+      jobs[q].c = dmaps.get() + at[q].second;
+    }
+    run_jobs(gather2d_kernel, jobs, s);
+  }
+  Uout = U;
+  Vdout = Vd;
+}
+
+}  // namespace cam

@@ -117,6 +117,11 @@ int cam_heff_apply(cam_heff* h, const double* in, double* out, int mem_kind, int
 /* Same matvec on vectors already in the device block layout (the Lanczos
  * layout), no gather/scatter; `in`/`out` device pointers of n complex. */
 int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int precision);
+/* Benchmark helper (no reference counterpart): run the matvec `reps` times
+ * on internal random block-layout vectors and return, per stage
+ * (L, W, W, R), the average launch duration in ms measured with HIP events
+ * on the context stream and the algorithmic flops of that stage. */
+int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, double* stage_flops);
 /* Reference order <-> device block layout (K3 gather/scatter). */
 int cam_heff_to_blocked(cam_heff* h, const double* in_dev, double* out_dev);
 int cam_heff_from_blocked(cam_heff* h, const double* in_dev, double* out_dev);
@@ -154,6 +159,23 @@ int cam_tebd_gate(cam_ctx* ctx, const cam_tensor* G, const cam_tensor* A0,
                   const cam_tensor* A1, const double* lambda0, int64_t n0, int64_t D,
                   double minS, cam_tensor** A0n, cam_tensor** A1n, double* lambda1,
                   int64_t cap, int64_t* n1, double* truncation);
+
+/* ---- iDMRG growth --------------------------------------------------------
+ * iDMRG::run / SuperBlock::grow_two_vertex (iDMRG_routines.cpp:19-33,
+ * DMRG_routines.cpp:309-374) with every tensor on the device: W the model
+ * MPO, target the target charges (nrules shorts).  converge=0 is the
+ * reference growth-step eigensolver setting (tol 0.1); mixed=1 runs the
+ * bulk of each eigensolve in FP32.  Each cam_idmrg_grow call adds two
+ * vertices; record[13] receives {N, energy, energy/N, entropy,
+ * truncation, chi kept, n, matvecs fp64, matvecs fp32, residual, seconds,
+ * heff flops, 0}. */
+typedef struct cam_idmrg_s cam_idmrg;
+int cam_idmrg_create(cam_ctx* ctx, const cam_tensor* W, const int16_t* target, int converge,
+                     int mixed, int ncv, cam_idmrg** out);
+int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record);
+/* singular values of the current central decomposition (capacity cap) */
+int cam_idmrg_values(const cam_idmrg* d, double* values, int64_t cap, int64_t* n);
+int cam_idmrg_destroy(cam_idmrg* d);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * Vector-Jacobian product of E = <psi|H_eff|psi>/<psi|psi>:

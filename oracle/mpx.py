@@ -59,7 +59,9 @@ def index_decompose(flat, dims):
     return out
 
 
-def index_compose(parts, dims):
+def index_compose(parts, dims, n=None):
+    if not parts:
+        return np.zeros(0 if n is None else n, dtype=np.int64)
     flat = np.zeros_like(np.asarray(parts[0], dtype=np.int64))
     mult = 1
     for p, d in zip(parts, dims):
@@ -246,8 +248,8 @@ def reshape_general(m, old_numrows, new_numrows, old_dims, new_order, conjugate)
     new_parts = [parts[o] for o in new_order]
     nr = _prod(new_dims[:new_numrows])
     nc = _prod(new_dims[new_numrows:])
-    r = index_compose(new_parts[:new_numrows], new_dims[:new_numrows]) if new_numrows else np.zeros(coo.nnz, np.int64)
-    c = index_compose(new_parts[new_numrows:], new_dims[new_numrows:])
+    r = index_compose(new_parts[:new_numrows], new_dims[:new_numrows], coo.nnz)
+    c = index_compose(new_parts[new_numrows:], new_dims[new_numrows:], coo.nnz)
     data = np.conj(coo.data) if conjugate else coo.data
     out = sp.csc_matrix((data, (r, c)), shape=(nr, nc))
     out.sort_indices()

@@ -1,0 +1,90 @@
+"""iTEBD restatement (oracle; test infrastructure only).
+
+Follows TEBD_routines.cpp:
+  * ``MakeOddBondHamiltonian`` / ``MakeEvenBondHamiltonian`` (826-867):
+    odd bond = row w-1 of W (all columns) times column 0 (all rows), even
+    bond = row w-1 / columns 1..w-2 times rows 1..w-2 / column 0, each
+    contracted on (3,1) and reordered 032415 with two row indices;
+  * ``MakeBondEvolutionOperator`` (869-880): exp(-i tau H_bond) from the
+    blockwise Hermitian eigendecomposition (Exponentiate,
+    sparse_interface.cpp:1707-1714), keeping the first four indices;
+  * ``TrotterDecomposition`` order 2 (41-51): (odd tau/2, even tau, odd tau);
+  * ``iTEBD::apply_and_decompose`` (233-266) and the unit-cell swap
+    (UnitCell.hpp:49).
+
+Parity pins (tests/test_oracle_tebd.py): exp(-i tau H) against
+scipy.linalg.expm of the dense bond Hamiltonian; unitarity; an exact
+two-site evolution without truncation (norm conservation, energy
+conservation of the single bond); the gauge-invariant two-site tensor.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+from .qn import MPXIndex
+from .mpx import MPX, contract, reorder, mpx_svd, diagonal_mpx
+from .dmrg import extract_mpo_block
+
+
+def odd_bond_hamiltonian(W: MPX) -> MPX:
+    w = W.indices[1].size
+    left = extract_mpo_block(W, (w - 1, w - 1), (0, W.indices[3].size - 1))
+    right = extract_mpo_block(W, (0, w - 1), (0, 0))
+    return reorder(contract(left, False, right, False, [(3, 1)]), False, [0, 3, 2, 4, 1, 5], 2)
+
+
+def even_bond_hamiltonian(W: MPX) -> MPX:
+    w = W.indices[1].size
+    left = extract_mpo_block(W, (w - 1, w - 1), (1, W.indices[3].size - 2))
+    right = extract_mpo_block(W, (1, w - 2), (0, 0))
+    return reorder(contract(left, False, right, False, [(3, 1)]), False, [0, 3, 2, 4, 1, 5], 2)
+
+
+def bond_evolution_operator(H: MPX, tau: float) -> MPX:
+    """exp(-i tau H) per charge block of the (d^2 x d^2) bond Hamiltonian,
+    indices = the first four of H."""
+    m = H.matrix.toarray()
+    groups = H.get_all_block_columns()
+    U = np.zeros_like(m)
+    for _, cols in groups:
+        blk = m[np.ix_(cols, cols)]
+        e, v = np.linalg.eigh(0.5 * (blk + blk.conj().T))
+        U[np.ix_(cols, cols)] = (v * np.exp(-1j * tau * e)) @ v.conj().T
+    s = sp.csc_matrix(U)
+    s.eliminate_zeros()
+    return MPX(H.rules, H.indices[:4], 2, s)
+
+
+def apply_and_decompose(G: MPX, A0: MPX, A1: MPX, lambda0, D=0, minS=0.0):
+    """Returns (new A0, new A1, new lambda0, truncation) after the unit-cell
+    swap; the other bond weight becomes the old lambda0."""
+    C = reorder(contract(G, False, contract(A0, False, A1, False, [(2, 1)]), False, [(2, 0), (3, 2)]),
+                False, [0, 2, 1, 3], 2)
+    lam = diagonal_mpx(C.rules, C.indices[3], C.indices[3], lambda0)
+    CL = contract(C, False, lam, False, [(3, 0)]).shift_num_row_indices(2)
+    d = mpx_svd(CL, D, minS)
+    U = d["U"]
+    A1p = reorder(contract(U, True, C, False, [(0, 0), (1, 1)]), False, [1, 0, 2], 2)
+    A1p.rescale(1.0 / np.sqrt(np.sum(np.asarray(lambda0) ** 2)))
+    return A1p, U, np.asarray(d["values"]), d["truncation"]
+
+
+def product_state_cell(rules, phys, level=0):
+    """Two-site unit cell of a product state in basis level `level`
+    (MakeProductStateUnitCell, UnitCell.cpp:190-210): A[s, a_l, a_r] with
+    1x1 bonds; the left bond carries charge 0, the right bond the level's
+    charge, repeated with accumulated charge on the second site."""
+    phys = np.asarray(phys, dtype=np.int64).reshape(-1, len(rules))
+    q = phys[level]
+    z = rules.zero(1)
+    q1 = q.reshape(1, -1)
+    q2 = rules.add(q, q).reshape(1, -1)
+    d = phys.shape[0]
+
+    def site(ql, qr):
+        idx = [MPXIndex(True, phys, True), MPXIndex(True, ql), MPXIndex(False, qr)]
+        T = np.zeros((d, 1, 1), complex)
+        T[level, 0, 0] = 1.0
+        return MPX.from_dense_tensor(rules, idx, 2, T)
+    return site(z, q1), site(q1, q2), np.array([1.0]), np.array([1.0])

@@ -1,0 +1,139 @@
+"""GPU parity: Lanczos eigensolve, blockwise SVD truncation, iTEBD gate.
+
+* lowest eigenpair (replacing arpack_eigs at TwoVertexComponents::HED,
+  DMRG_routines.cpp:835-896): energy within 1e-10 of the oracle's dense
+  eigh, eigenvector overlap 1 - 1e-10; FP32-mixed solve to the same bar;
+* SVD (MPX_matrix.cpp:157-192, sparse_interface.cpp:1248-1406): singular
+  values within 1e-12, identical multiset of new-bond charges, identical
+  kept length / discarded weight under D and max_trunc, reconstruction
+  and orthonormality; phase gauge: phases of singular vectors are free,
+  so U and V^dagger are compared through U S V^dagger;
+* iTEBD gate (TEBD_routines.cpp:233-266): new weights and truncation, and
+  the gauge-invariant two-site tensor, against the oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import to_lib, from_lib, rel_err
+from oracle.qn import ChargeRules, MPXIndex
+from oracle import models, dmrg, tebd
+from oracle import mpx as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _heis_comp(steps=5, chi=24):
+    rules, phys, W = models.heisenberg_mpo()
+    idm = dmrg.IDMRG(W, converge=True)
+    for _ in range(steps):
+        idm.grow(chi)
+    LB = dmrg.make_left_block(idm.LB, W, idm.U)
+    RB = dmrg.make_right_block(idm.RB, W, idm.Vd)
+    return rules, LB, W, RB, idm
+
+
+def _chain_comp(d=8, chi=16, steps=3, seed=0):
+    rules, phys, W, E, sig = models.coupled_chain_model(d=d, seed=seed)
+    idm = dmrg.IDMRG(W, converge=True)
+    for _ in range(steps):
+        idm.grow(chi)
+    LB = dmrg.make_left_block(idm.LB, W, idm.U)
+    RB = dmrg.make_right_block(idm.RB, W, idm.Vd)
+    return rules, LB, W, RB, idm
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("model", ["heisenberg", "chain"])
+def test_lowest_eigenpair(ctx, model, mixed):
+    print(f"case lowest {model} mixed={mixed}", flush=True)
+    import chainamps_amd as cam
+    rules, LB, W, RB, idm = _heis_comp() if model == "heisenberg" else _chain_comp()
+    comp = dmrg.TwoVertexComponents(LB, W, RB)
+    Hd = comp.dense_matrix()
+    w, v = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
+    H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
+    E, vec, info = H.lowest(converge=True, ncv=20, mixed=mixed)
+    assert abs(E - w[0]) <= 1e-10 * max(1.0, abs(w[0])), (E, w[0], info)
+    ov = abs(np.vdot(v[:, 0], vec)) / np.linalg.norm(vec)
+    assert ov >= 1 - 1e-10, (ov, info)
+    if mixed:
+        assert info["matvecs_fp32"] > 0
+    # gauge of DMRG_routines.cpp:870-890: the largest entry is real positive
+    i = np.argmax(np.abs(vec))
+    assert vec[i].imag == 0 and vec[i].real > 0
+
+
+def test_lowest_loose_with_guess(ctx):
+    """Growth-step setting (converge=0: tol 0.1) from the oracle's own
+    prediction vector: one Krylov pass, energy close to the converged one."""
+    print("case lowest_loose", flush=True)
+    import chainamps_amd as cam
+    rules, LB, W, RB, idm = _heis_comp(steps=6, chi=32)
+    comp = dmrg.TwoVertexComponents(LB, W, RB)
+    H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
+    E0, _, _ = H.lowest(converge=True)
+    rng = np.random.default_rng(0)
+    g = rng.standard_normal(H.n) + 0j
+    E, vec, info = H.lowest(guess=g, converge=False)
+    assert info["matvecs_fp64"] <= 21 and abs(E - E0) < 1e-2 * abs(E0)
+    # a zero guess falls back to a random start (the info=-9 path)
+    E2, _, _ = H.lowest(guess=np.zeros(H.n, complex), converge=True)
+    assert abs(E2 - E0) < 1e-10
+
+
+def _psi_tensor(rules, comp, vec):
+    return dmrg.psi_from_vector(comp, vec)
+
+
+@pytest.mark.parametrize("D,trunc", [(0, 0.0), (20, 0.0), (0, 1e-4), (7, 1e-9)])
+def test_svd_truncate(ctx, D, trunc):
+    print(f"case svd D={D} trunc={trunc}", flush=True)
+    import chainamps_amd as cam
+    rules, LB, W, RB, idm = _chain_comp(d=8, chi=16, steps=3)
+    comp = dmrg.TwoVertexComponents(LB, W, RB)
+    rng = np.random.default_rng(4)
+    # a random wavefunction with a decaying spectrum
+    vec = (rng.standard_normal(comp.n) + 1j * rng.standard_normal(comp.n)) * np.exp(-0.05 * np.arange(comp.n))
+    psi = _psi_tensor(rules, comp, vec)
+    ref = O.mpx_svd(psi, D, trunc)
+    U, vals, Vd, disc = cam.svd_truncate(to_lib(ctx, psi), D, trunc)
+    assert len(vals) == len(ref["values"])
+    assert np.allclose(vals, ref["values"], rtol=1e-12, atol=1e-14)
+    assert abs(disc - ref["truncation"]) <= 1e-12 * max(1.0, ref["truncation"])
+    Ul, Vl = from_lib(U, rules), from_lib(Vd, rules)
+    assert Ul.is_consistent() and Vl.is_consistent()
+    # new bond charges: identical multiset, and identical sequence when the
+    # spectrum has no exact ties
+    assert sorted(map(tuple, Ul.indices[-1].charges)) == sorted(map(tuple, ref["U"].indices[-1].charges))
+    if len(np.unique(ref["values"])) == len(ref["values"]):
+        assert np.array_equal(Ul.indices[-1].charges, ref["U"].indices[-1].charges)
+    Um, Vm = Ul.matrix.toarray(), Vl.matrix.toarray()
+    rec = Um @ np.diag(vals) @ Vm
+    rec_ref = ref["U"].matrix.toarray() @ np.diag(ref["values"]) @ ref["Vd"].matrix.toarray()
+    assert rel_err(rec, rec_ref) <= 1e-11
+    assert np.allclose(Um.conj().T @ Um, np.eye(len(vals)), atol=1e-12)
+
+
+def test_itebd_gate(ctx):
+    print("case itebd_gate", flush=True)
+    import chainamps_amd as cam
+    rules, phys, W, E, sig = models.coupled_chain_model(d=6, seed=3, Jperp=0.3)
+    Godd = tebd.bond_evolution_operator(tebd.odd_bond_hamiltonian(W), 0.05)
+    Geven = tebd.bond_evolution_operator(tebd.even_bond_hamiltonian(W), 0.1)
+    A0, A1, l0, l1 = tebd.product_state_cell(rules, phys, 0)
+    tA0, tA1 = to_lib(ctx, A0), to_lib(ctx, A1)
+    tl0 = l0
+    for step, G in enumerate([Godd, Geven, Godd, Geven, Godd]):
+        nA0, nA1, nl0, tr = tebd.apply_and_decompose(G, A0, A1, l0, D=12, minS=1e-12)
+        gA0, gA1, gl0, gtr = cam.tebd_gate(to_lib(ctx, G), tA0, tA1, tl0, D=12, minS=1e-12)
+        assert len(gl0) == len(nl0)
+        assert np.allclose(gl0, nl0, rtol=1e-10, atol=1e-13)
+        assert abs(gtr - tr) <= 1e-10
+        # gauge-invariant: the two-site tensor A1' (x) U with weights
+        two_ref = O.contract(nA1, False, O.contract(nA0, False, O.diagonal_mpx(rules, nA0.indices[2], nA0.indices[2], l0), False, [(2, 0)]), False, [(2, 1)])
+        gA0m, gA1m = from_lib(gA0, rules), from_lib(gA1, rules)
+        two = O.contract(gA1m, False, O.contract(gA0m, False, O.diagonal_mpx(rules, gA0m.indices[2], gA0m.indices[2], tl0), False, [(2, 0)]), False, [(2, 1)])
+        assert rel_err(two.to_dense_tensor(), two_ref.to_dense_tensor()) <= 1e-10
+        # continue both evolutions from their own states (weights: unit-cell swap)
+        A0, A1, l0 = nA0, nA1, nl0
+        tA0, tA1, tl0 = gA0, gA1, gl0
