@@ -57,7 +57,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
   constexpr int PB = TN * KC / 256;   // B elements per thread per chunk
   constexpr int FM = TM / 32;         // 16-row fragments per wave
   constexpr int FN = TN / 32;
-  __shared__ R sAr[KC * LDA], sAi[KC * LDA], sBr[KC * LDB], sBi[KC * LDB];
+  // two LDS buffers: chunk c+1 is stored while chunk c is multiplied, so one
+  // barrier per chunk suffices
+  __shared__ R sAr[2][KC * LDA], sAi[2][KC * LDA], sBr[2][KC * LDB], sBi[2][KC * LDB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -143,7 +145,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
         }
       }
     };
-    auto store = [&](const GemmSeg& G) {
+    auto store = [&](const GemmSeg& G, int buf) {
+      R* __restrict__ pAr = sAr[buf];
+      R* __restrict__ pAi = sAi[buf];
+      R* __restrict__ pBr = sBr[buf];
+      R* __restrict__ pBi = sBi[buf];
       const bool cA = G.flags & kSegConjA, cB = G.flags & kSegConjB, al = G.flags & kSegAlpha;
       const R are = static_cast<R>(G.alpha_re), aim = static_cast<R>(G.alpha_im);
       const bool akc = G.flags & kSegAKContig;
@@ -157,34 +163,40 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
           xi = xr * aim + xi * are;
           xr = tr;
         }
-        sAr[k * LDA + m] = xr;
-        sAi[k * LDA + m] = xi;
+        pAr[k * LDA + m] = xr;
+        pAi[k * LDA + m] = xi;
       }
       if (G.flags & kSegBNContig) {
 #pragma unroll
         for (int p = 0; p < PB; ++p) {
           const int e = tid + p * 256;
           const int n = e % TN, k = e / TN;
-          sBr[k * LDB + n] = rb[p].x;
-          sBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
+          pBr[k * LDB + n] = rb[p].x;
+          pBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
         }
       } else {
 #pragma unroll
         for (int p = 0; p < PB; ++p) {
           const int e = tid + p * 256;
           const int k = e % KC, n = e / KC;
-          sBr[k * LDB + n] = rb[p].x;
-          sBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
+          pBr[k * LDB + n] = rb[p].x;
+          pBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
         }
       }
     };
 
-    if (s < send) load(S, 0);
+    int buf = 0;
+    if (s < send) {
+      load(S, 0);
+      store(S, 0);
+    }
+    __syncthreads();
     while (s < send) {
-      __syncthreads();
-      store(S);
-      __syncthreads();
-      // advance (segment, k0) and prefetch the next chunk
+      const R* __restrict__ cAr = sAr[buf];
+      const R* __restrict__ cAi = sAi[buf];
+      const R* __restrict__ cBr = sBr[buf];
+      const R* __restrict__ cBi = sBi[buf];
+      // advance (segment, k0) and prefetch the next chunk into registers
       int ns = s, nk = k0 + KC;
       GemmSeg NS = S;
       if (nk >= S.K) {
@@ -200,24 +212,41 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
         R ar[FM], ai[FM], br[FN], bi[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          ar[i] = sAr[kr * LDA + wm * (TM / 2) + i * 16 + fr];
-          ai[i] = sAi[kr * LDA + wm * (TM / 2) + i * 16 + fr];
+          ar[i] = cAr[kr * LDA + wm * (TM / 2) + i * 16 + fr];
+          ai[i] = cAi[kr * LDA + wm * (TM / 2) + i * 16 + fr];
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          br[j] = sBr[kr * LDB + wn * (TN / 2) + j * 16 + fr];
-          bi[j] = sBi[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+          br[j] = cBr[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+          bi[j] = cBi[kr * LDB + wn * (TN / 2) + j * 16 + fr];
         }
+        // four passes over the fragments so that consecutive MFMAs never
+        // depend on each other (each accumulator is revisited 2*FM*FN
+        // instructions later)
+        R nai[FM];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) nai[i] = -ai[i];
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            accr[i][j] = Mf<R>::mma(ar[i], br[j], accr[i][j]);
-            accr[i][j] = Mf<R>::mma(-ai[i], bi[j], accr[i][j]);
-            acci[i][j] = Mf<R>::mma(ar[i], bi[j], acci[i][j]);
-            acci[i][j] = Mf<R>::mma(ai[i], br[j], acci[i][j]);
-          }
+          for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(ar[i], br[j], accr[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ar[i], bi[j], acci[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(nai[i], bi[j], accr[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ai[i], br[j], acci[i][j]);
       }
+      // the other buffer was last read before the previous barrier
+      if (ns < send) store(NS, buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
       s = ns;
       k0 = nk;
       S = NS;

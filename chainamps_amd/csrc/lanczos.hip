@@ -260,7 +260,9 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
     const double nx = L.norm(L.x.get());
     launch_scale(cplx{1.0 / nx, 0.0}, L.x.get(), n, s);
   }
-  const int max_restarts = converge ? 200 : 100;
+  // ARPACK's maxiter: 50 implicit restarts converged, 2 for growth steps
+  // (arpack_interface.cpp:28); explicit restarts need more when converging
+  const int max_restarts = converge ? 200 : 2;
   double theta = 0.0, resid = 0.0, prev_resid = 1e300;
   int restarts = 0;
   bool use32 = mixed;
@@ -272,7 +274,10 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
     auto c0 = L.orthogonalize(1, L.w.get());
     theta = c0[0].real();
     resid = L.norm(L.w.get());
-    if (resid <= tol_eff * std::max(std::fabs(theta), 1e-300) || restarts >= max_restarts) break;
+    // like ARPACK, convergence is judged only once a full Krylov space has
+    // been built: a good start vector still gets one cycle of ncv vectors
+    if (restarts >= 1 && (resid <= tol_eff * std::max(std::fabs(theta), 1e-300) || restarts >= max_restarts))
+      break;
     if (use32 && restarts > 1 && (resid > 0.5 * prev_resid || resid < 1e3 * eps * std::fabs(theta)))
       use32 = false;   // final FP64 Lanczos steps
     prev_resid = resid;

@@ -58,6 +58,11 @@ def test_vjp_psi_and_W_fd(ctx):
         assert abs(dW[k].real - fd) <= 1e-6 * max(1.0, abs(fd)), (k, dW[k], fd)
 
 
+def _model_W(d, J, E=None):
+    rules, phys, W, E0, sig = models.coupled_chain_model(d=d, seed=2, Jperp=J, E=E)
+    return W, E0
+
+
 def test_hellmann_feynman_couplings(ctx):
     print("case hellmann_feynman", flush=True)
     import chainamps_amd as cam
@@ -69,36 +74,36 @@ def test_hellmann_feynman_couplings(ctx):
     m = W.matrix.tocsc()
     m.sort_indices()
     dpsi, dW = H.vjp(psi, m.nnz)
-    # dW/dJ: only the block W[1,0] = J*sigma depends on J
-    rows = np.repeat(np.arange(m.shape[1]), np.diff(m.indptr))   # column of each entry
-    col_of = rows
+    # at the ground state dE/dconj(psi) vanishes
+    assert np.linalg.norm(dpsi) <= 1e-9
+    col_of = np.repeat(np.arange(m.shape[1]), np.diff(m.indptr))
     row_of = m.indices
-    bl = row_of // d
-    br = col_of // d
+    bl, br = row_of // d, col_of // d
+    # chain rule: only W[1,0] = J*sigma depends on J
     dWdJ = np.where((bl == 1) & (br == 0), m.data / J, 0)
-    dEdJ = float(np.sum(dW * dWdJ).real)
-    # expectation value <psi| dH_eff/dJ |psi> with W' = dW/dJ
-    Wd = W.copy()
-    Wd.matrix = m.copy()
-    Wd.matrix.data = dWdJ.astype(complex)
-    hf = _energy(LB, Wd, RB, psi)
-    # the MPO enters both vertices: dH/dJ has one J-term per vertex
-    assert abs(dEdJ - hf) <= 1e-9 * max(1.0, abs(hf))
-    # central difference of the lowest eigenvalue (environments fixed)
-    h = 1e-5
+    dEdJ = np.sum(dW * dWdJ)
+    assert abs(dEdJ.imag) <= 1e-10
+    dEdJ = dEdJ.real
+    # Hellmann-Feynman expectation <psi|dH_eff/dJ|psi>: E(psi; J) is
+    # quadratic in J at fixed psi, so the central difference is exact
+    h = 1e-3
+    hf = (_energy(LB, _model_W(d, J + h)[0], RB, psi) - _energy(LB, _model_W(d, J - h)[0], RB, psi)) / (2 * h)
+    assert abs(dEdJ - hf) <= 1e-9 * max(1.0, abs(hf)), (dEdJ, hf)
+    # central finite difference of the lowest eigenvalue (environments fixed)
+    hh = 1e-5
+
     def lowest(Jv):
-        _, _, Wj, _, _ = models.coupled_chain_model(d=d, seed=2, Jperp=Jv)
-        Hj = cam.Heff(to_lib(ctx, LB), to_lib(ctx, Wj), to_lib(ctx, RB))
-        return Hj.lowest(converge=True)[0]
-    fd = (lowest(J + h) - lowest(J - h)) / (2 * h)
-    assert abs(dEdJ - fd) <= 1e-6 * max(1.0, abs(fd))
-    # dE/dE_n: the on-site block W[2,0] = diag(E)
+        Wj, _ = _model_W(d, Jv)
+        return cam.Heff(to_lib(ctx, LB), to_lib(ctx, Wj), to_lib(ctx, RB)).lowest(converge=True)[0]
+    fd = (lowest(J + hh) - lowest(J - hh)) / (2 * hh)
+    assert abs(dEdJ - fd) <= 1e-6 * max(1.0, abs(fd)), (dEdJ, fd)
+    # dE/dE_n for every level: chain rule through W[2,0] = diag(E), against
+    # the (exact) central difference of E(psi; E_n)
     for nlev in range(d):
         k = np.where((bl == 2) & (br == 0) & (row_of % d == nlev) & (col_of % d == nlev))[0]
         assert len(k) == 1
-        Wn = W.copy()
-        Wn.matrix = m.copy()
-        Wn.matrix.data = np.zeros_like(m.data)
-        Wn.matrix.data[k[0]] = 1.0
-        hf_n = _energy(LB, Wn, RB, psi)
-        assert abs(dW[k[0]].real - hf_n) <= 1e-10 * max(1.0, abs(hf_n))
+        Ep, Em = E.copy(), E.copy()
+        Ep[nlev] += h
+        Em[nlev] -= h
+        ref = (_energy(LB, _model_W(d, J, Ep)[0], RB, psi) - _energy(LB, _model_W(d, J, Em)[0], RB, psi)) / (2 * h)
+        assert abs(dW[k[0]].real - ref) <= 1e-9 * max(1.0, abs(ref)), (nlev, dW[k[0]], ref)

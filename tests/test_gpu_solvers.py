@@ -85,16 +85,41 @@ def _psi_tensor(rules, comp, vec):
     return dmrg.psi_from_vector(comp, vec)
 
 
-@pytest.mark.parametrize("D,trunc", [(0, 0.0), (20, 0.0), (0, 1e-4), (7, 1e-9)])
+def _theta_with_spectrum(rules, comp, rng, spectra):
+    """psi whose column blocks have prescribed singular values: each QN
+    column block is U diag(s) V^dagger with random unitaries."""
+    psi = dmrg.psi_from_vector(comp, np.ones(comp.n))
+    m = psi.matrix.toarray()
+    out = np.zeros_like(m)
+    for gi, (_, cols) in enumerate(psi.get_all_block_columns()):
+        rows = np.nonzero(np.abs(m[:, cols]).sum(axis=1))[0]
+        if len(rows) == 0:
+            continue
+        k = min(len(rows), len(cols))
+        s = spectra(gi, k)
+        qa, _ = np.linalg.qr(rng.standard_normal((len(rows), k)) + 1j * rng.standard_normal((len(rows), k)))
+        qb, _ = np.linalg.qr(rng.standard_normal((len(cols), k)) + 1j * rng.standard_normal((len(cols), k)))
+        out[np.ix_(rows, cols)] = (qa * s) @ qb.conj().T
+    return O.MPX(rules, psi.indices, 2, out)
+
+
+@pytest.mark.parametrize("D,trunc", [(0, 0.0), (20, 0.0), (0, 1e-4), (7, 1e-9), (13, 0.0)])
 def test_svd_truncate(ctx, D, trunc):
     print(f"case svd D={D} trunc={trunc}", flush=True)
     import chainamps_amd as cam
     rules, LB, W, RB, idm = _chain_comp(d=8, chi=16, steps=3)
     comp = dmrg.TwoVertexComponents(LB, W, RB)
     rng = np.random.default_rng(4)
-    # a random wavefunction with a decaying spectrum
-    vec = (rng.standard_normal(comp.n) + 1j * rng.standard_normal(comp.n)) * np.exp(-0.05 * np.arange(comp.n))
-    psi = _psi_tensor(rules, comp, vec)
+    # distinct, well separated values in every block; a pair of equal values
+    # across the two parity blocks (a tie up to rounding), and a
+    # near-degenerate pair (relative gap 1e-5) straddling D=13
+    def spectra(gi, k):
+        s = np.exp(-0.35 * np.arange(k) - 0.11 * gi)
+        if k > 8:
+            s[3] = np.exp(-0.35 * 3)          # equal across blocks
+            s[6] = s[5] * (1 - 1e-5) if gi == 0 else s[6]
+        return s
+    psi = _theta_with_spectrum(rules, comp, rng, spectra)
     ref = O.mpx_svd(psi, D, trunc)
     U, vals, Vd, disc = cam.svd_truncate(to_lib(ctx, psi), D, trunc)
     assert len(vals) == len(ref["values"])
@@ -102,38 +127,42 @@ def test_svd_truncate(ctx, D, trunc):
     assert abs(disc - ref["truncation"]) <= 1e-12 * max(1.0, ref["truncation"])
     Ul, Vl = from_lib(U, rules), from_lib(Vd, rules)
     assert Ul.is_consistent() and Vl.is_consistent()
-    # new bond charges: identical multiset, and identical sequence when the
-    # spectrum has no exact ties
-    assert sorted(map(tuple, Ul.indices[-1].charges)) == sorted(map(tuple, ref["U"].indices[-1].charges))
-    if len(np.unique(ref["values"])) == len(ref["values"]):
-        assert np.array_equal(Ul.indices[-1].charges, ref["U"].indices[-1].charges)
+    # new bond charges: identical sequence, except inside clusters of equal
+    # values (relative gap < 1e-12), whose order is set by the last bits of
+    # two different LAPACK implementations: there the multisets agree
+    qg, qr = Ul.indices[-1].charges[:, 0], ref["U"].indices[-1].charges[:, 0]
+    rv = np.asarray(ref["values"])
+    start = 0
+    for i in range(1, len(rv) + 1):
+        if i == len(rv) or abs(rv[i] - rv[i - 1]) > 1e-12 * rv[i - 1]:
+            assert sorted(qg[start:i]) == sorted(qr[start:i]), (start, i)
+            start = i
     Um, Vm = Ul.matrix.toarray(), Vl.matrix.toarray()
     rec = Um @ np.diag(vals) @ Vm
     rec_ref = ref["U"].matrix.toarray() @ np.diag(ref["values"]) @ ref["Vd"].matrix.toarray()
     assert rel_err(rec, rec_ref) <= 1e-11
     assert np.allclose(Um.conj().T @ Um, np.eye(len(vals)), atol=1e-12)
+    assert np.allclose(Vm @ Vm.conj().T, np.eye(len(vals)), atol=1e-12)
 
 
 def test_itebd_gate(ctx):
+    """One apply_and_decompose from identical inputs at every step of a
+    second-order Trotter sequence; the gauge-invariant output is the
+    two-site tensor U (x) A1' over the new bond (= P_U C / |lambda0|)."""
     print("case itebd_gate", flush=True)
     import chainamps_amd as cam
     rules, phys, W, E, sig = models.coupled_chain_model(d=6, seed=3, Jperp=0.3)
     Godd = tebd.bond_evolution_operator(tebd.odd_bond_hamiltonian(W), 0.05)
     Geven = tebd.bond_evolution_operator(tebd.even_bond_hamiltonian(W), 0.1)
     A0, A1, l0, l1 = tebd.product_state_cell(rules, phys, 0)
-    tA0, tA1 = to_lib(ctx, A0), to_lib(ctx, A1)
-    tl0 = l0
-    for step, G in enumerate([Godd, Geven, Godd, Geven, Godd]):
+    for step, G in enumerate([Godd, Geven, Godd, Geven, Godd, Geven]):
         nA0, nA1, nl0, tr = tebd.apply_and_decompose(G, A0, A1, l0, D=12, minS=1e-12)
-        gA0, gA1, gl0, gtr = cam.tebd_gate(to_lib(ctx, G), tA0, tA1, tl0, D=12, minS=1e-12)
-        assert len(gl0) == len(nl0)
-        assert np.allclose(gl0, nl0, rtol=1e-10, atol=1e-13)
-        assert abs(gtr - tr) <= 1e-10
-        # gauge-invariant: the two-site tensor A1' (x) U with weights
-        two_ref = O.contract(nA1, False, O.contract(nA0, False, O.diagonal_mpx(rules, nA0.indices[2], nA0.indices[2], l0), False, [(2, 0)]), False, [(2, 1)])
-        gA0m, gA1m = from_lib(gA0, rules), from_lib(gA1, rules)
-        two = O.contract(gA1m, False, O.contract(gA0m, False, O.diagonal_mpx(rules, gA0m.indices[2], gA0m.indices[2], tl0), False, [(2, 0)]), False, [(2, 1)])
-        assert rel_err(two.to_dense_tensor(), two_ref.to_dense_tensor()) <= 1e-10
-        # continue both evolutions from their own states (weights: unit-cell swap)
+        gA0, gA1, gl0, gtr = cam.tebd_gate(to_lib(ctx, G), to_lib(ctx, A0), to_lib(ctx, A1), l0, D=12, minS=1e-12)
+        assert len(gl0) == len(nl0), step
+        assert np.allclose(gl0, nl0, rtol=1e-10, atol=1e-13), step
+        assert abs(gtr - tr) <= 1e-10, step
+        ga0, ga1 = from_lib(gA0, rules), from_lib(gA1, rules)
+        two_ref = O.contract(nA1, False, nA0, False, [(2, 1)])
+        two = O.contract(ga1, False, ga0, False, [(2, 1)])
+        assert rel_err(two.to_dense_tensor(), two_ref.to_dense_tensor()) <= 1e-10, step
         A0, A1, l0 = nA0, nA1, nl0
-        tA0, tA1, tl0 = gA0, gA1, gl0
