@@ -139,7 +139,7 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
     RBd = make_dummy_right_block(*ctx, *W, target);
     H.build(*ctx, LBd, W, RBd);
     DevBuf<cplx> x(std::max<int64_t>(H.n, 1));
-    E = lanczos_lowest(H, x.get(), false, true, 0.0, ncv, false, info);
+    E = eigs_lowest(H, x.get(), false, true, 0.0, ncv, false, info);
     TensorPtr psi = H.tensor_from_vector(x.get());
     const int64_t bond = (chi < 1 && trunc <= 0.0) ? W->idx[0].size : chi;
     double disc = 0.0;
@@ -163,7 +163,7 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
       H.vector_from_tensor(*guess, x.get());
       have = true;
     }
-    E = lanczos_lowest(H, x.get(), have, converge, 0.0, ncv, mixed, info);
+    E = eigs_lowest(H, x.get(), have, converge, 0.0, ncv, mixed, info);
     TensorPtr psi = H.tensor_from_vector(x.get());
     double disc = 0.0;
     svd_truncate(*ctx, *psi, chi, trunc, U, Vd, values, disc);

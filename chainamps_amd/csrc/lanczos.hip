@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <limits>
 
 #include "solvers.hpp"
@@ -65,6 +66,26 @@ __global__ void random_fill_kernel(double2* x, int64_t n, uint64_t seed) {
   }
 }
 
+// Lanczos step tail on the device: beta = |w| from the norm slot; the step
+// breaks down when beta <= 16 eps max(1, |alpha|), alpha = Re of the summed
+// CGS2 coefficients c1[j] + c2[j].  Writes v_{j+1} = w / beta (zero after a
+// breakdown) and the decision {beta, flag} for the host.
+__global__ void lanczos_next_kernel(const double2* __restrict__ w, double2* __restrict__ next, int64_t n,
+                                    const double2* __restrict__ nrm, const double2* __restrict__ c1,
+                                    const double2* __restrict__ c2, double eps16, double2* __restrict__ dec) {
+  const double alpha = c1->x + c2->x;
+  const double beta = sqrt(fmax(0.0, nrm->x));
+  const bool brk = beta <= eps16 * fmax(1.0, fabs(alpha));
+  const double inv = brk ? 0.0 : 1.0 / beta;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dec = make_double2(beta, brk ? 1.0 : 0.0);
+  if (!next) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = w[i];
+    next[i] = make_double2(v.x * inv, v.y * inv);
+  }
+}
+
 struct Lanczos {
   Heff& H;
   hipStream_t s;
@@ -73,8 +94,11 @@ struct Lanczos {
   DevBuf<cplx> V;             // (m+1) x n
   DevBuf<cplx> w, x, y;       // work
   DevBuf<cplx> coef, partial;
+  DevBuf<cplx> C1, C2, Nrm, Dec;   // per-step CGS2 coefficients, norms, decisions
   DevBuf<double> dD, dE, dC;
   DevBuf<int> dinfo;
+  hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
+  bool use_graphs = true;
   int mv64 = 0, mv32 = 0;
 
   Lanczos(Heff& h, int ncv) : H(h), s(h.ctx->stream), n(h.n) {
@@ -86,12 +110,91 @@ struct Lanczos {
     y.alloc(n);
     coef.alloc(64);
     partial.alloc(static_cast<size_t>(multi_dot_blocks(n)) * 64);
+    C1.alloc(64 * 64);
+    C2.alloc(64 * 64);
+    Nrm.alloc(64);
+    Dec.alloc(64);
     dD.alloc(64);
     dE.alloc(64);
     dC.alloc(64 * 64);
     dinfo.alloc(1);
+    const char* e = std::getenv("CAM_NO_GRAPH");
+    use_graphs = !(e && *e && *e != '0');
   }
+  ~Lanczos() {
+    for (auto& g : cycle_exec)
+      if (g) (void)hipGraphExecDestroy(g);
+  }
+  Lanczos(const Lanczos&) = delete;
+  Lanczos& operator=(const Lanczos&) = delete;
   cplx* col(int j) { return V.get() + static_cast<int64_t>(j) * n; }
+
+  // Steps j = 1 .. m-1 of one Krylov cycle with every scalar kept on the
+  // device (col(1) must hold the normalised residual of step 0): matvec,
+  // two CGS passes, norm, normalisation.  No host synchronisation, so the
+  // whole sequence is one hipGraph.
+  void enqueue_cycle(int prec) {
+    const double eps16 = 16 * std::numeric_limits<double>::epsilon();
+    for (int j = 1; j < m; ++j) {
+      H.apply(col(j), w.get(), prec);
+      cplx* c1 = C1.get() + j * 64;
+      cplx* c2 = C2.get() + j * 64;
+      launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c1, s);
+      launch_multi_axpy(V.get(), n, j + 1, c1, w.get(), n, s);
+      launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c2, s);
+      launch_multi_axpy(V.get(), n, j + 1, c2, w.get(), n, s);
+      launch_multi_dot(w.get(), n, 1, w.get(), n, partial.get(), Nrm.get() + j, s);
+      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)));
+      hipLaunchKernelGGL(lanczos_next_kernel, dim3(grid), dim3(256), 0, s,
+                         reinterpret_cast<const double2*>(w.get()),
+                         j + 1 < m ? reinterpret_cast<double2*>(col(j + 1)) : nullptr, n,
+                         reinterpret_cast<const double2*>(Nrm.get() + j),
+                         reinterpret_cast<const double2*>(c1 + j), reinterpret_cast<const double2*>(c2 + j),
+                         eps16, reinterpret_cast<double2*>(Dec.get() + j));
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  // runs the cycle (captured once per precision, then replayed) and returns
+  // alpha_1.., beta_1.. up to the first breakdown
+  bool run_cycle(int prec, std::vector<double>& alpha, std::vector<double>& beta, int& k) {
+    if (m > 1) {
+      hipGraphExec_t& ge = cycle_exec[prec];
+      if (!ge && use_graphs) {
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+          enqueue_cycle(prec);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(s, &g));
+        const hipError_t st = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_CHECK(st);
+      }
+      if (ge) HIP_CHECK(hipGraphLaunch(ge, s));
+      else enqueue_cycle(prec);
+    }
+    std::vector<cplx> c1(64 * 64), c2(64 * 64), dec(64);
+    HIP_CHECK(hipMemcpyAsync(c1.data(), C1.get(), c1.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(c2.data(), C2.get(), c2.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(dec.data(), Dec.get(), dec.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    bool invariant = false;
+    for (int j = 1; j < m; ++j) {
+      if (prec == 0) ++mv64; else ++mv32;
+      alpha.push_back(c1[j * 64 + j].re + c2[j * 64 + j].re);
+      k = j + 1;
+      if (dec[j].im != 0.0) {
+        invariant = true;
+        break;
+      }
+      beta.push_back(dec[j].re);
+    }
+    return invariant;
+  }
 
   void matvec(const cplx* in, cplx* out, int prec) {
     H.apply(in, out, prec);
@@ -244,6 +347,7 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
   // converged criterion is taken as max(tol, 64 eps)
   tol_eff = std::max(tol_eff, 64 * eps);
   ncv = ncv > 0 ? ncv : 20;
+  if (mixed) H.prepare_fp32();   // one-time conversions stay out of the captured cycle
   Lanczos L(H, ncv);
   // start vector (SparseVectorWithRestriction -> DumbExtractWithZeros); a
   // zero guess takes the info=-9 path: random start
@@ -288,22 +392,7 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
       beta.push_back(resid);
       launch_scale(cplx{1.0 / resid, 0.0}, L.w.get(), n, s);
       launch_copy(L.w.get(), L.col(1), n, s);
-      for (int j = 1; j < m; ++j) {
-        L.matvec(L.col(j), L.w.get(), use32 ? 1 : 0);
-        auto c = L.orthogonalize(j + 1, L.w.get());
-        alpha.push_back(c[j].real());
-        const double b = L.norm(L.w.get());
-        k = j + 1;
-        if (b <= 16 * eps * std::max(1.0, std::fabs(alpha.back()))) {
-          invariant = true;
-          break;
-        }
-        beta.push_back(b);
-        if (j + 1 < m) {
-          launch_scale(cplx{1.0 / b, 0.0}, L.w.get(), n, s);
-          launch_copy(L.w.get(), L.col(j + 1), n, s);
-        }
-      }
+      invariant = L.run_cycle(use32 ? 1 : 0, alpha, beta, k);
     }
     std::vector<double> svec;
     beta.resize(k, 0.0);
@@ -336,6 +425,73 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
   return theta;
 }
 
+namespace {
+
+__global__ void unit_vector_kernel(double2* x, int64_t n, int64_t j) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+}
+
+// TranslationBlock<DenseMatrix>::TranslateRows cut (sparse_interface.hpp:414-425):
+// entries with |x| < SPARSETOL are not stored
+__global__ void cut_small_kernel(const double2* __restrict__ x, double2* __restrict__ y, int64_t n, double tol) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = x[i];
+    y[i] = (sqrt(v.x * v.x + v.y * v.y) >= tol) ? v : make_double2(0.0, 0.0);
+  }
+}
+
+}  // namespace
+
+// Dense branch of TwoVertexComponents::HED (DMRG_routines.cpp:850-858): the
+// whole H_eff is formed (column j = H_eff e_j, in the block layout, which is
+// a permutation of the reference's allowed-index order) and diagonalised
+// with rocSOLVER zheevd; the lowest eigenvector is returned with entries
+// below SPARSETOL cut and no phase gauge, as the reference's LAPACK path.
+double dense_lowest(Heff& H, cplx* xout, double* info) {
+  const int64_t n = H.n;
+  CAM_REQUIRE(n > 0 && n <= kSparseThreshold, "dense eigensolver: size out of range");
+  hipStream_t s = H.ctx->stream;
+  DevBuf<cplx> A(static_cast<size_t>(n) * n), e(n);
+  DevBuf<double> D(n), off(n);
+  DevBuf<int> dinfo(1);
+  const int grid = static_cast<int>((n + 255) / 256);
+  for (int64_t j = 0; j < n; ++j) {
+    hipLaunchKernelGGL(unit_vector_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<double2*>(e.get()), n, j);
+    HIP_CHECK(hipGetLastError());
+    H.apply(e.get(), A.get() + j * n, 0);
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(H.ctx->blas_handle());
+  const rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(n),
+                                             reinterpret_cast<rocblas_double_complex*>(A.get()),
+                                             static_cast<rocblas_int>(n), D.get(), off.get(), dinfo.get());
+  CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheevd failed");
+  hipLaunchKernelGGL(cut_small_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const double2*>(A.get()),
+                     reinterpret_cast<double2*>(xout), n, std::numeric_limits<double>::epsilon());
+  HIP_CHECK(hipGetLastError());
+  double e0 = 0.0;
+  int inf = 0;
+  HIP_CHECK(hipMemcpyAsync(&e0, D.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&inf, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  CAM_REQUIRE(inf == 0, "dense eigensolver did not converge");
+  if (info) {
+    info[0] = static_cast<double>(n);
+    info[1] = 0.0;
+    info[2] = 0.0;
+    info[3] = 0.0;
+  }
+  return e0;
+}
+
+double eigs_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv, bool mixed,
+                   double* info) {
+  if (H.n <= kSparseThreshold) return dense_lowest(H, x, info);
+  return lanczos_lowest(H, x, have_start, converge, tol, ncv, mixed, info);
+}
+
 void heff_lowest(Heff& H, const cplx* guess_ref, bool converge, double tol, int ncv, bool mixed,
                  double* eval, cplx* evec_ref, double* info) {
   const int64_t n = H.n;
@@ -346,7 +502,7 @@ void heff_lowest(Heff& H, const cplx* guess_ref, bool converge, double tol, int 
     y.upload(guess_ref, n, s);
     launch_scatter_map(y.get(), x.get(), H.perm.get(), n, s);
   }
-  *eval = lanczos_lowest(H, x.get(), guess_ref != nullptr, converge, tol, ncv, mixed, info);
+  *eval = eigs_lowest(H, x.get(), guess_ref != nullptr, converge, tol, ncv, mixed, info);
   if (evec_ref) {
     launch_gather_map(x.get(), y.get(), H.perm.get(), n, s);
     HIP_CHECK(hipMemcpyAsync(evec_ref, y.get(), n * sizeof(cplx), hipMemcpyDeviceToHost, s));

@@ -15,6 +15,12 @@ void heff_lowest(Heff& H, const cplx* guess_ref, bool converge, double tol, int 
 // and receives the gauge-fixed eigenvector; returns the eigenvalue.
 double lanczos_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv,
                       bool mixed, double* info);
+// Dense branch for n <= SPARSE_THRESHOLD (sparse_interface.hpp:16) and the
+// dispatcher that picks it, as TwoVertexComponents::HED does.
+constexpr int64_t kSparseThreshold = 1000;
+double dense_lowest(Heff& H, cplx* x, double* info);
+double eigs_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv, bool mixed,
+                   double* info);
 void gauge_fix(cplx* x, int64_t n, hipStream_t s);
 // contract with a diagonal on index k (values per position of that index)
 void scale_along_index(Ctx& ctx, BSTensor& T, int k, const std::vector<double>& lam);

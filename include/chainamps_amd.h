@@ -133,8 +133,12 @@ int cam_heff_from_blocked(cam_heff* h, const double* in_dev, double* out_dev);
  * converge=1 tolerance `tol` (<=0: machine precision).  ncv: Krylov size
  * (reference 20).  mixed=1 runs the bulk in FP32 with FP64 residual
  * correction and final FP64 steps.  evec: host, reference order, phase
- * gauge of DMRG_routines.cpp:870-890 applied.  info (may be NULL) receives
- * {matvecs_fp64, matvecs_fp32, restarts, residual_norm}. */
+ * gauge of DMRG_routines.cpp:870-890 applied.  As in the reference, an
+ * H_eff of dimension n <= SPARSE_THRESHOLD (1000) is instead formed densely
+ * and diagonalised (DMRG_routines.cpp:850-858): guess, tol, ncv and mixed
+ * are unused, the eigenvector has entries below machine epsilon cut and no
+ * phase gauge.  info (may be NULL) receives {matvecs_fp64, matvecs_fp32,
+ * restarts, residual_norm}. */
 int cam_heff_lowest(cam_heff* h, const double* guess, int converge, double tol,
                     int ncv, int mixed, double* eval, double* evec, double* info);
 

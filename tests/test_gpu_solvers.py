@@ -47,8 +47,10 @@ def _chain_comp(d=8, chi=16, steps=3, seed=0):
 def test_lowest_eigenpair(ctx, model, mixed):
     print(f"case lowest {model} mixed={mixed}", flush=True)
     import chainamps_amd as cam
-    rules, LB, W, RB, idm = _heis_comp() if model == "heisenberg" else _chain_comp()
+    # n = 1008 and 8192: above SPARSE_THRESHOLD, so the Krylov path runs
+    rules, LB, W, RB, idm = _heis_comp(steps=6, chi=32) if model == "heisenberg" else _chain_comp()
     comp = dmrg.TwoVertexComponents(LB, W, RB)
+    assert comp.n > 1000
     Hd = comp.dense_matrix()
     w, v = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
     H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
@@ -61,6 +63,27 @@ def test_lowest_eigenpair(ctx, model, mixed):
     # gauge of DMRG_routines.cpp:870-890: the largest entry is real positive
     i = np.argmax(np.abs(vec))
     assert vec[i].imag == 0 and vec[i].real > 0
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_lowest_dense_branch(ctx, mixed):
+    """n <= SPARSE_THRESHOLD (1000): H_eff is formed densely and
+    diagonalised (DMRG_routines.cpp:850-858) -- one FP64 matvec per unit
+    vector, no FP32 work, eigenvector cut at |x| < eps (no phase gauge)."""
+    print(f"case lowest_dense mixed={mixed}", flush=True)
+    import chainamps_amd as cam
+    rules, LB, W, RB, idm = _heis_comp()
+    comp = dmrg.TwoVertexComponents(LB, W, RB)
+    assert comp.n <= 1000
+    Hd = comp.dense_matrix()
+    w, v = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
+    H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
+    E, vec, info = H.lowest(converge=False, mixed=mixed)
+    assert abs(E - w[0]) <= 1e-10 * max(1.0, abs(w[0]))
+    assert abs(abs(np.vdot(v[:, 0], vec)) - 1.0) <= 1e-10
+    assert info["matvecs_fp64"] == comp.n and info["matvecs_fp32"] == 0
+    nz = vec[vec != 0]
+    assert np.all(np.abs(nz) >= np.finfo(float).eps)
 
 
 def test_lowest_loose_with_guess(ctx):
