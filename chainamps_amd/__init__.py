@@ -7,7 +7,9 @@ CPU implementation -- a missing library is an ImportError-like failure at
 first use (``load_library``).
 """
 from .capi import (CamError, Context, Tensor, Heff, IDMRG, contract, reorder, svd_truncate,
-                   tebd_gate, load_library, library_path, CAM_FP64, CAM_FP32)
+                   tebd_gate, extract_mpo_block, bond_hamiltonian, bond_evolution_operator,
+                   load_library, library_path, CAM_FP64, CAM_FP32)
 
 __all__ = ["CamError", "Context", "Tensor", "Heff", "IDMRG", "contract", "reorder", "svd_truncate",
-           "tebd_gate", "load_library", "library_path", "CAM_FP64", "CAM_FP32"]
+           "tebd_gate", "extract_mpo_block", "bond_hamiltonian", "bond_evolution_operator",
+           "load_library", "library_path", "CAM_FP64", "CAM_FP32"]

@@ -27,6 +27,9 @@ COMMON = [
     "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: no per-chunk copies between VGPRs and AGPRs
+    # around the grouped GEMM's K loop
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
     f"-I{ROCM}/include",
 ]

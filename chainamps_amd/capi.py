@@ -375,6 +375,36 @@ def tebd_gate(G: Tensor, A0: Tensor, A1: Tensor, lambda0, D=0, minS=0.0):
     return Tensor(ctx, a), Tensor(ctx, b), l1[:n1.value].copy(), tr.value
 
 
+_SIGS_GATES = {
+    "cam_extract_mpo_block": [_p, _p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.POINTER(_p)],
+    "cam_bond_hamiltonian": [_p, _p, C.c_int, C.POINTER(_p)],
+    "cam_bond_evolution_operator": [_p, _p, C.c_double, C.POINTER(_p)],
+}
+_SIGS.update(_SIGS_GATES)
+
+
+def extract_mpo_block(W: Tensor, rows, cols) -> Tensor:
+    """MPO_matrix::ExtractMPOBlock: rows=(r0, r1), cols=(c0, c1) inclusive."""
+    h = _p()
+    _check(_LIB.cam_extract_mpo_block(W.ctx.h, W.h, int(rows[0]), int(rows[1]), int(cols[0]), int(cols[1]),
+                                      C.byref(h)))
+    return Tensor(W.ctx, h)
+
+
+def bond_hamiltonian(W: Tensor, odd=True) -> Tensor:
+    """MakeOddBondHamiltonian / MakeEvenBondHamiltonian."""
+    h = _p()
+    _check(_LIB.cam_bond_hamiltonian(W.ctx.h, W.h, int(bool(odd)), C.byref(h)))
+    return Tensor(W.ctx, h)
+
+
+def bond_evolution_operator(H: Tensor, timestep) -> Tensor:
+    """MakeBondEvolutionOperator: exp(-i timestep H) per charge block."""
+    h = _p()
+    _check(_LIB.cam_bond_evolution_operator(H.ctx.h, H.h, float(timestep), C.byref(h)))
+    return Tensor(H.ctx, h)
+
+
 _SIGS_IDMRG = {
     "cam_idmrg_create": [_p, _p, C.POINTER(C.c_int16), C.c_int, C.c_int, C.c_int, C.POINTER(_p)],
     "cam_idmrg_grow": [_p, C.c_int64, C.c_double, _dp],

@@ -198,6 +198,28 @@ int cam_reorder(cam_ctx* c, const cam_tensor* A, int conjA, int n, const int64_t
   CAM_API_END
 }
 
+int cam_extract_mpo_block(cam_ctx* c, const cam_tensor* W, int64_t r0, int64_t r1, int64_t c0, int64_t c1,
+                          cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  *out = new cam_tensor{extract_mpo_block(c->ctx, *W->t, r0, r1, c0, c1), c};
+  CAM_API_END
+}
+
+int cam_bond_hamiltonian(cam_ctx* c, const cam_tensor* W, int odd, cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  *out = new cam_tensor{bond_hamiltonian(c->ctx, *W->t, odd != 0), c};
+  CAM_API_END
+}
+
+int cam_bond_evolution_operator(cam_ctx* c, const cam_tensor* H, double timestep, cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  *out = new cam_tensor{bond_evolution_operator(c->ctx, *H->t, timestep), c};
+  CAM_API_END
+}
+
 int cam_heff_create(cam_ctx* c, const cam_tensor* L, const cam_tensor* W, const cam_tensor* R,
                     cam_heff** out, int64_t* n) {
   CAM_API_BEGIN

@@ -24,6 +24,32 @@ namespace {
 constexpr int KC = 16;     // K chunk staged in LDS
 constexpr int PADL = 4;    // LDS row padding (elements)
 
+// global-address-space access (global_load / global_store rather than
+// flat) through native vector types: HIP's double2/float2 have reference
+// constructors that would cast the address back to the flat space
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <typename T> struct NV;
+template <> struct NV<double2> { using T = d2v; };
+template <> struct NV<float2> { using T = f2v; };
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  using V = typename NV<T>::T;
+  const V v = *(const __attribute__((address_space(1))) V*)(p);
+  T r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, const T& a) {
+  using V = typename NV<T>::T;
+  V v;
+  v.x = a.x;
+  v.y = a.y;
+  *(__attribute__((address_space(1))) V*)(p) = v;
+}
+
 template <typename R> struct CT;
 template <> struct CT<double> { using T = double2; };
 template <> struct CT<float> { using T = float2; };
@@ -83,6 +109,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       }
 
     C ra[PA], rb[PB];
+    uint32_t amask = 0, bmask = 0;   // in-bounds flags of the staged elements
     const int send = P.seg0 + P.nseg;
     int s = P.seg0;
     while (s < send && segs[s].K <= 0) ++s;
@@ -90,6 +117,10 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
     GemmSeg S;
     if (s < send) S = segs[s];
 
+    // operand loads: global address space (not flat, whose completion the
+    // LDS waits of the MFMA loop would also have to wait for) and branch-free:
+    // a padding element reads the block's first element and is zeroed when
+    // it is staged to LDS, so nothing waits on the loads before the MFMAs
     auto load = [&](const GemmSeg& G, int kk0) {
       const int asp = (G.flags >> 8) & 15, bsp = (G.flags >> 12) & 15;
       const C* A = (asp ? reinterpret_cast<const C*>(bases.p[asp]) + reinterpret_cast<intptr_t>(G.A)
@@ -98,51 +129,26 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       const C* B = (bsp ? reinterpret_cast<const C*>(bases.p[bsp]) + reinterpret_cast<intptr_t>(G.B)
                         : reinterpret_cast<const C*>(G.B)) +
                    static_cast<int64_t>(tl.batch) * G.bsB;
-      if (G.flags & kSegAKContig) {
+      const bool akc = G.flags & kSegAKContig, bnc = G.flags & kSegBNContig;
 #pragma unroll
-        for (int p = 0; p < PA; ++p) {
-          const int e = tid + p * 256;
-          const int k = e % KC, m = e / KC;
-          const int gm = m0 + m, gk = kk0 + k;
-          C v;
-          v.x = 0; v.y = 0;
-          if (gm < M && gk < G.K) v = A[gk + static_cast<int64_t>(gm) * G.csA];
-          ra[p] = v;
-        }
-      } else {
-#pragma unroll
-        for (int p = 0; p < PA; ++p) {
-          const int e = tid + p * 256;
-          const int m = e % TM, k = e / TM;
-          const int gm = m0 + m, gk = kk0 + k;
-          C v;
-          v.x = 0; v.y = 0;
-          if (gm < M && gk < G.K) v = A[gm + static_cast<int64_t>(gk) * G.csA];
-          ra[p] = v;
-        }
+      for (int p = 0; p < PA; ++p) {
+        const int e = tid + p * 256;
+        const int m = akc ? e / KC : e % TM, k = akc ? e % KC : e / TM;
+        const int gm = m0 + m, gk = kk0 + k;
+        const bool ok = gm < M && gk < G.K;
+        const int64_t off = akc ? gk + static_cast<int64_t>(gm) * G.csA : gm + static_cast<int64_t>(gk) * G.csA;
+        ra[p] = gload(A + (ok ? off : 0));
+        amask = ok ? (amask | (1u << p)) : (amask & ~(1u << p));
       }
-      if (G.flags & kSegBNContig) {
 #pragma unroll
-        for (int p = 0; p < PB; ++p) {
-          const int e = tid + p * 256;
-          const int n = e % TN, k = e / TN;
-          const int gn = n0 + n, gk = kk0 + k;
-          C v;
-          v.x = 0; v.y = 0;
-          if (gn < N && gk < G.K) v = B[static_cast<int64_t>(gk) * G.ldB + gn];
-          rb[p] = v;
-        }
-      } else {
-#pragma unroll
-        for (int p = 0; p < PB; ++p) {
-          const int e = tid + p * 256;
-          const int k = e % KC, n = e / KC;
-          const int gn = n0 + n, gk = kk0 + k;
-          C v;
-          v.x = 0; v.y = 0;
-          if (gn < N && gk < G.K) v = B[gk + static_cast<int64_t>(gn) * G.ldB];
-          rb[p] = v;
-        }
+      for (int p = 0; p < PB; ++p) {
+        const int e = tid + p * 256;
+        const int n = bnc ? e % TN : e / KC, k = bnc ? e / TN : e % KC;
+        const int gn = n0 + n, gk = kk0 + k;
+        const bool ok = gn < N && gk < G.K;
+        const int64_t off = bnc ? static_cast<int64_t>(gk) * G.ldB + gn : gk + static_cast<int64_t>(gn) * G.ldB;
+        rb[p] = gload(B + (ok ? off : 0));
+        bmask = ok ? (bmask | (1u << p)) : (bmask & ~(1u << p));
       }
     };
     auto store = [&](const GemmSeg& G, int buf) {
@@ -157,7 +163,8 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       for (int p = 0; p < PA; ++p) {
         const int e = tid + p * 256;
         const int m = akc ? e / KC : e % TM, k = akc ? e % KC : e / TM;
-        R xr = ra[p].x, xi = cA ? -ra[p].y : ra[p].y;
+        const bool ok = (amask >> p) & 1u;
+        R xr = ok ? ra[p].x : R(0), xi = ok ? (cA ? -ra[p].y : ra[p].y) : R(0);
         if (al) {
           const R tr = xr * are - xi * aim;
           xi = xr * aim + xi * are;
@@ -171,16 +178,18 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
         for (int p = 0; p < PB; ++p) {
           const int e = tid + p * 256;
           const int n = e % TN, k = e / TN;
-          pBr[k * LDB + n] = rb[p].x;
-          pBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
+          const bool ok = (bmask >> p) & 1u;
+          pBr[k * LDB + n] = ok ? rb[p].x : R(0);
+          pBi[k * LDB + n] = ok ? (cB ? -rb[p].y : rb[p].y) : R(0);
         }
       } else {
 #pragma unroll
         for (int p = 0; p < PB; ++p) {
           const int e = tid + p * 256;
           const int k = e % KC, n = e / KC;
-          pBr[k * LDB + n] = rb[p].x;
-          pBi[k * LDB + n] = cB ? -rb[p].y : rb[p].y;
+          const bool ok = (bmask >> p) & 1u;
+          pBr[k * LDB + n] = ok ? rb[p].x : R(0);
+          pBi[k * LDB + n] = ok ? (cB ? -rb[p].y : rb[p].y) : R(0);
         }
       }
     };
@@ -272,11 +281,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
             val.x = accr[i][j][v];
             val.y = acci[i][j][v];
             if (P.beta) {
-              const C old = *dst;
+              const C old = gload(dst);
               val.x += old.x;
               val.y += old.y;
             }
-            *dst = val;
+            gstore(dst, val);
           }
         }
       }

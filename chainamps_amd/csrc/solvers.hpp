@@ -34,6 +34,12 @@ void tebd_gate(Ctx& ctx, const BSTensor& G, const BSTensor& A0, const BSTensor& 
                const std::vector<double>& lambda0, int64_t D, double minS, TensorPtr& A0n,
                TensorPtr& A1n, std::vector<double>& lambda1, double& truncation);
 
+// Gate construction: ExtractMPOBlock, odd/even bond Hamiltonians,
+// exp(-i timestep H) per charge block.
+TensorPtr extract_mpo_block(Ctx& ctx, const BSTensor& W, int64_t r0, int64_t r1, int64_t c0, int64_t c1);
+TensorPtr bond_hamiltonian(Ctx& ctx, const BSTensor& W, bool odd);
+TensorPtr bond_evolution_operator(Ctx& ctx, const BSTensor& H, double timestep);
+
 // dE/dconj(psi) and dE/dW for E = <psi|H|psi>/<psi|psi>.
 void heff_vjp(Heff& H, const cplx* psi_ref, cplx* dpsi_ref, cplx* dW, int64_t nnzW);
 

@@ -164,6 +164,21 @@ int cam_tebd_gate(cam_ctx* ctx, const cam_tensor* G, const cam_tensor* A0,
                   double minS, cam_tensor** A0n, cam_tensor** A1n, double* lambda1,
                   int64_t cap, int64_t* n1, double* truncation);
 
+/* ---- gate construction ---------------------------------------------------
+ * MPO_matrix::ExtractMPOBlock (MPO_matrix.cpp:69-85): rows [r0, r1] of the
+ * left MPO bond, columns [c0, c1] of the right one. */
+int cam_extract_mpo_block(cam_ctx* ctx, const cam_tensor* W, int64_t r0, int64_t r1, int64_t c0,
+                          int64_t c1, cam_tensor** out);
+/* MakeOddBondHamiltonian (odd=1) / MakeEvenBondHamiltonian (odd=0)
+ * (TEBD_routines.cpp:826-867): [s1', s2', s1, s2, dummy, dummy], two row
+ * indices. */
+int cam_bond_hamiltonian(cam_ctx* ctx, const cam_tensor* W, int odd, cam_tensor** out);
+/* MakeBondEvolutionOperator (TEBD_routines.cpp:869-880): exp(-i timestep H)
+ * from the blockwise Hermitian decomposition (Exponentiate,
+ * sparse_interface.cpp:1707-1714; SparseMatrix::HED 1414-1441); the result
+ * keeps the first four indices of H. */
+int cam_bond_evolution_operator(cam_ctx* ctx, const cam_tensor* H, double timestep, cam_tensor** out);
+
 /* ---- iDMRG growth --------------------------------------------------------
  * iDMRG::run / SuperBlock::grow_two_vertex (iDMRG_routines.cpp:19-33,
  * DMRG_routines.cpp:309-374) with every tensor on the device: W the model
