@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--chi", type=int, default=500)
     ap.add_argument("--d", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eig", action="store_true", help="skip the growth-step eigensolve timing")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -132,7 +133,7 @@ def main():
 
     ctx = cam.Context(0)
     t0 = time.perf_counter()
-    L, W, R = workloads.coupled_chain_heff_inputs(ctx, chi=a.chi, d=a.d, seed=0, Jperp=0.1)
+    L, W, R = workloads.coupled_chain_heff_inputs(ctx, chi=a.chi, d=a.d, seed=0, Jperp=0.1, hermitian=True)
     H = cam.Heff(L, W, R)
     setup_s = time.perf_counter() - t0
     n = H.n
@@ -164,6 +165,27 @@ def main():
     dom = int(np.argmax(ms64))
     achieved = fl[dom] / (ms64[dom] * 1e-3) / 1e12
     stage_names = ["L", "W1", "W2", "R"]
+    # eigensolve of one iDMRG growth step (the reference's setting for it:
+    # converge=0, i.e. tol 0.1 and one Krylov cycle of ncv=20 vectors,
+    # arpack_interface.cpp:28-39) from the same start vector, FP64 against
+    # FP32-mixed (FP64 first step and reorthogonalisation, FP32 matvecs)
+    eig = {}
+    free, _ = torch.cuda.mem_get_info()
+    need = 23 * 16 * n          # Krylov basis (ncv+1 vectors) and work vector, FP64
+    if free < 1.05 * need:
+        eig = {"skipped": f"needs {need / 1e9:.0f} GB of free device memory, {free / 1e9:.0f} GB free"}
+    elif not a.no_eig:
+        for mixed, name in ((False, "fp64"), (True, "mixed")):
+            y.copy_(x)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            E, info = H.lowest_device(y.data_ptr(), have_start=True, converge=False, ncv=20, mixed=mixed)
+            el = time.perf_counter() - t
+            eig[name] = dict(seconds=round(el, 3), energy=E, matvecs_fp64=info["matvecs_fp64"],
+                             matvecs_fp32=info["matvecs_fp32"], residual=info["residual"])
+        eig["speedup_mixed_vs_fp64"] = round(eig["fp64"]["seconds"] / eig["mixed"]["seconds"], 3)
+        eig["note"] = ("Lanczos eigensolve with the growth-step setting; the SVD and the block updates of "
+                       "the step are not included")
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_budget)
     line = {
         "metric": "H_eff matvec TFLOP/s + iDMRG step time, chi=2000 d=64, FP32-mixed vs FP64",
@@ -190,6 +212,9 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4), "traffic": None,
                      "kernel": f"grouped_gemm_kernel<double> stage {stage_names[dom]}"},
         "cpu_baseline": cpu,
+        "growth_step_eigensolve": eig or None,
+        "autotuned_launch": {"fields": "(algo 0=4M/1=3M, tile_mode, wg_per_cu) per stage L, W1, W2, R",
+                             "fp64": H.get_launch(cam.CAM_FP64), "fp32": H.get_launch(cam.CAM_FP32)},
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(line))

@@ -72,9 +72,12 @@ _SIGS = {
     "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
+    "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int],
+    "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
     "cam_heff_to_blocked": [_p, _p, _p],
     "cam_heff_from_blocked": [_p, _p, _p],
     "cam_heff_lowest": [_p, _p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _p, _dp],
+    "cam_heff_lowest_blocked": [_p, _p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp],
     "cam_svd_truncate": [_p, _p, C.c_int64, C.c_double, C.POINTER(_p), C.POINTER(_p), _dp, C.c_int64,
                          _i64p, _dp],
     "cam_tebd_gate": [_p, _p, _p, _p, _dp, C.c_int64, C.c_int64, C.c_double, C.POINTER(_p),
@@ -310,6 +313,16 @@ class Heff:
         _check(_LIB.cam_heff_profile(self.h, precision, reps, _ptr(ms, C.c_double), _ptr(fl, C.c_double)))
         return ms, fl
 
+    def set_launch(self, precision=CAM_FP64, algo=1, tile_mode=1, wg_per_cu=4):
+        """Fix the stage GEMM launch configuration (disables autotuning)."""
+        _check(_LIB.cam_heff_set_launch(self.h, precision, algo, tile_mode, wg_per_cu))
+
+    def get_launch(self, precision=CAM_FP64):
+        """[(algo, tile_mode, wg_per_cu)] for the stages L, W, W, R."""
+        cfg = np.zeros(12, np.int32)
+        _check(_LIB.cam_heff_get_launch(self.h, precision, _ptr(cfg, C.c_int)))
+        return [tuple(int(v) for v in cfg[3 * i:3 * i + 3]) for i in range(4)]
+
     def to_blocked(self, din, dout):
         _check(_LIB.cam_heff_to_blocked(self.h, din, dout))
 
@@ -328,6 +341,16 @@ class Heff:
                                     int(mixed), C.byref(ev), vec.ctypes.data, _ptr(info, C.c_double)))
         return ev.value, vec, dict(matvecs_fp64=int(info[0]), matvecs_fp32=int(info[1]),
                                    restarts=int(info[2]), residual=float(info[3]))
+
+    def lowest_device(self, dx, have_start=False, converge=True, tol=0.0, ncv=20, mixed=False):
+        """Eigensolve on a device block-layout vector (pointer `dx`, n
+        complex128), overwritten by the eigenvector; returns (E, info)."""
+        ev = C.c_double()
+        info = np.zeros(8, np.float64)
+        _check(_LIB.cam_heff_lowest_blocked(self.h, C.c_void_p(dx), int(have_start), int(converge), tol, ncv,
+                                            int(mixed), C.byref(ev), _ptr(info, C.c_double)))
+        return ev.value, dict(matvecs_fp64=int(info[0]), matvecs_fp32=int(info[1]),
+                              restarts=int(info[2]), residual=float(info[3]))
 
     def vjp(self, psi, nnzW):
         psi = np.ascontiguousarray(psi, dtype=np.complex128)

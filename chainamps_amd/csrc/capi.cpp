@@ -276,6 +276,33 @@ int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, dou
   CAM_API_END
 }
 
+int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  CAM_REQUIRE(algo == kAlgo4M || algo == kAlgo3M, "algo must be 0 (four real products) or 1 (three)");
+  CAM_REQUIRE(tile_mode >= 0 && tile_mode < kTileModes, "tile_mode out of range");
+  CAM_REQUIRE(wg_per_cu >= 1 && wg_per_cu <= 64, "wg_per_cu out of range");
+  LaunchCfg c;
+  c.algo = algo;
+  c.tile_mode = tile_mode;
+  c.wg_per_cu = wg_per_cu;
+  h->h.set_launch(precision, c);
+  CAM_API_END
+}
+
+int cam_heff_get_launch(const cam_heff* h, int precision, int* cfg) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  for (int i = 0; i < 4; ++i) {
+    const LaunchCfg& c = h->h.st[i].cfg[precision];
+    cfg[3 * i] = c.algo;
+    cfg[3 * i + 1] = c.tile_mode;
+    cfg[3 * i + 2] = c.wg_per_cu;
+  }
+  CAM_API_END
+}
+
 int cam_heff_to_blocked(cam_heff* h, const double* in_dev, double* out_dev) {
   CAM_API_BEGIN
   set_device(h->owner);
@@ -330,6 +357,15 @@ int cam_heff_lowest(cam_heff* h, const double* guess, int converge, double tol, 
   set_device(h->owner);
   heff_lowest(h->h, reinterpret_cast<const cplx*>(guess), converge != 0, tol, ncv, mixed != 0, eval,
               reinterpret_cast<cplx*>(evec), info);
+  CAM_API_END
+}
+
+int cam_heff_lowest_blocked(cam_heff* h, double* x, int have_start, int converge, double tol, int ncv, int mixed,
+                            double* eval, double* info) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(x != nullptr, "cam_heff_lowest_blocked: null vector");
+  *eval = eigs_lowest(h->h, reinterpret_cast<cplx*>(x), have_start != 0, converge != 0, tol, ncv, mixed != 0, info);
   CAM_API_END
 }
 

@@ -5,15 +5,23 @@
 // MFMA fragments.  K is consumed in chunks of 16 staged through LDS with
 // real and imaginary parts in separate planes (so one MFMA operand is one
 // LDS read); the next chunk is prefetched into registers while the current
-// one is multiplied.  A complex product is four real MFMAs:
-//   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre.
+// one is multiplied.  A complex product is four real MFMAs,
+//   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre,
+// or three (Gauss): P1 = Are*Bre, P2 = Aim*Bim, P3 = (Are+Aim)*(Bre+Bim),
+//   Cre = P1 - P2,   Cim = P3 - P1 - P2.
 //
 // MFMA 16x16x4 operand layout (wave64): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16].  Accumulator element v of lane l is D[row][l%16] with
 // row = l/16 + 4v for the f64 instruction and row = 4(l/16) + v for f32
 // (measured on gfx950).
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <unordered_map>
 
 #include "gemm.hpp"
 
@@ -70,7 +78,10 @@ template <> struct Mf<float> {
   static __device__ __forceinline__ int row(int fk, int v) { return 4 * fk + v; }
 };
 
-template <typename R, int TM, int TN>
+// G3: complex products with three real MFMAs (P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2) instead of four;
+// the error stays O(eps |A||B|) per product, as for the four-product form.
+template <typename R, int TM, int TN, bool G3>
 __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __restrict__ probs,
                                                            const GemmSeg* __restrict__ segs,
                                                            const GemmTile* __restrict__ tiles,
@@ -99,13 +110,16 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
     const int M = P.M, N = P.N;
     const int m0 = tl.m0, n0 = tl.n0;
 
-    acc_t accr[FM][FN], acci[FM][FN];
+    // four-product form: accr = re, acci = im; G3: accr = P1, acci = P2,
+    // accs = P3
+    acc_t accr[FM][FN], acci[FM][FN], accs[G3 ? FM : 1][G3 ? FN : 1];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         accr[i][j] = acc_t{0, 0, 0, 0};
         acci[i][j] = acc_t{0, 0, 0, 0};
+        if (G3) accs[G3 ? i : 0][G3 ? j : 0] = acc_t{0, 0, 0, 0};
       }
 
     C ra[PA], rb[PB];
@@ -229,28 +243,49 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
           br[j] = cBr[kr * LDB + wn * (TN / 2) + j * 16 + fr];
           bi[j] = cBi[kr * LDB + wn * (TN / 2) + j * 16 + fr];
         }
-        // four passes over the fragments so that consecutive MFMAs never
-        // depend on each other (each accumulator is revisited 2*FM*FN
-        // instructions later)
-        R nai[FM];
+        if constexpr (G3) {
+          // three passes; an accumulator is revisited FM*FN MFMAs later
+          R sa[FM], sb[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i) nai[i] = -ai[i];
+          for (int i = 0; i < FM; ++i) sa[i] = ar[i] + ai[i];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+          for (int j = 0; j < FN; ++j) sb[j] = br[j] + bi[j];
 #pragma unroll
-          for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(ar[i], br[j], accr[i][j]);
+          for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(ar[i], br[j], accr[i][j]);
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ar[i], bi[j], acci[i][j]);
+          for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ai[i], bi[j], acci[i][j]);
 #pragma unroll
-          for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(nai[i], bi[j], accr[i][j]);
+          for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int j = 0; j < FN; ++j) accs[i][j] = Mf<R>::mma(sa[i], sb[j], accs[i][j]);
+        } else {
+          // four passes over the fragments so that consecutive MFMAs never
+          // depend on each other (each accumulator is revisited 2*FM*FN
+          // instructions later)
+          R nai[FM];
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ai[i], br[j], acci[i][j]);
+          for (int i = 0; i < FM; ++i) nai[i] = -ai[i];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(ar[i], br[j], accr[i][j]);
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ar[i], bi[j], acci[i][j]);
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) accr[i][j] = Mf<R>::mma(nai[i], bi[j], accr[i][j]);
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ai[i], br[j], acci[i][j]);
+        }
       }
       // the other buffer was last read before the previous barrier
       if (ns < send) store(NS, buf ^ 1);
@@ -278,8 +313,14 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
           if (m < M) {
             C* dst = Cp + m + static_cast<int64_t>(n) * P.csC;
             C val;
-            val.x = accr[i][j][v];
-            val.y = acci[i][j][v];
+            if constexpr (G3) {
+              const R p1 = accr[i][j][v], p2 = acci[i][j][v];
+              val.x = p1 - p2;
+              val.y = accs[i][j][v] - p1 - p2;
+            } else {
+              val.x = accr[i][j][v];
+              val.y = acci[i][j][v];
+            }
             if (P.beta) {
               const C old = gload(dst);
               val.x += old.x;
@@ -294,15 +335,20 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
 }
 
 template <typename R, int TM, int TN>
-void launch_shape(const GemmPlan& plan, int shape, hipStream_t s, int grid_limit,
+void launch_shape(const GemmPlan& plan, const LaunchCfg& c, int shape, hipStream_t s, int grid_limit,
                   const GemmBases& bases) {
-  const int nt = static_cast<int>(plan.tiles[shape].size());
+  const int nt = static_cast<int>(plan.tiles[c.tile_mode][shape].size());
   if (!nt) return;
   static int cus = device_cu_count();
-  int grid = std::min(nt, cus * 4);
+  int grid = std::min(nt, cus * std::max(1, c.wg_per_cu));
   if (grid_limit > 0) grid = std::min(grid, grid_limit);
-  hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN>), dim3(grid), dim3(256), 0, s,
-                     plan.d_probs.get(), plan.d_segs.get(), plan.d_tiles[shape].get(), nt, bases);
+  const GemmTile* tiles = plan.d_tiles[c.tile_mode][shape].get();
+  if (c.algo == kAlgo3M)
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, true>), dim3(grid), dim3(256), 0, s,
+                       plan.d_probs.get(), plan.d_segs.get(), tiles, nt, bases);
+  else
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, false>), dim3(grid), dim3(256), 0, s,
+                       plan.d_probs.get(), plan.d_segs.get(), tiles, nt, bases);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -332,6 +378,7 @@ int GemmPlan::add_problem(void* C, int64_t csC, int M, int N, int batch, int64_t
   p.cspace = cspace;
   probs.push_back(p);
   uploaded = false;
+  for (int m = 0; m < kTileModes; ++m) built[m] = tiles_uploaded[m] = false;
   return static_cast<int>(probs.size()) - 1;
 }
 
@@ -359,84 +406,201 @@ void GemmPlan::add_segment(int prob, const void* A, int64_t csA, int64_t bsA, co
 }
 
 void GemmPlan::choose_tiles(int mode) {
-  for (auto& p : probs) {
-    if (mode == 0) { p.tile = 0; continue; }
-    double best = 1e300;
-    int bi = 0;
-    for (int t = 0; t < kNumTileShapes; ++t) {
-      const double tm = kTileShapes[t].tm, tn = kTileShapes[t].tn;
-      const double pm = std::ceil(p.M / tm) * tm, pn = std::ceil(p.N / tn) * tn;
-      // padded area, with a small per-tile overhead so bigger tiles win ties
-      const double cost = pm * pn * (1.0 + 0.02 * (4096.0 / (tm * tn)));
-      if (cost < best) { best = cost; bi = t; }
-    }
-    p.tile = bi;
-  }
-  uploaded = false;
+  CAM_REQUIRE(mode >= 0 && mode < kTileModes, "unknown tile mode");
+  cfg[0].tile_mode = cfg[1].tile_mode = mode;
 }
 
 void GemmPlan::build_tiles() {
-  for (int t = 0; t < kNumTileShapes; ++t) tiles[t].clear();
+  build_tiles_mode(cfg[0].tile_mode);
+  if (cfg[1].tile_mode != cfg[0].tile_mode) build_tiles_mode(cfg[1].tile_mode);
+}
+
+void GemmPlan::build_tiles_mode(int mode) {
+  auto& T = tiles[mode];
+  for (int t = 0; t < kNumTileShapes; ++t) T[t].clear();
   std::vector<double> cost[kNumTileShapes];
   for (size_t pi = 0; pi < probs.size(); ++pi) {
     const GemmProb& p = probs[pi];
+    int shape = 0;
+    if (mode != 0) {
+      double best = 1e300;
+      for (int t = (mode == 2 ? 1 : 0); t < kNumTileShapes; ++t) {
+        const double tm = kTileShapes[t].tm, tn = kTileShapes[t].tn;
+        const double pm = std::ceil(p.M / tm) * tm, pn = std::ceil(p.N / tn) * tn;
+        // padded area, with a small per-tile overhead so bigger tiles win ties
+        const double c = pm * pn * (1.0 + 0.02 * (4096.0 / (tm * tn)));
+        if (c < best) { best = c; shape = t; }
+      }
+    }
     int64_t ktot = 0;
     for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) ktot += (segs[s].K + KC - 1) / KC * KC;
     if (ktot == 0 && p.beta) continue;   // nothing to add
-    const int tm = kTileShapes[p.tile].tm, tn = kTileShapes[p.tile].tn;
+    const int tm = kTileShapes[shape].tm, tn = kTileShapes[shape].tn;
     for (int b = 0; b < p.batch; ++b)
       for (int n0 = 0; n0 < p.N; n0 += tn)
         for (int m0 = 0; m0 < p.M; m0 += tm) {
-          tiles[p.tile].push_back(GemmTile{static_cast<int32_t>(pi), b, m0, n0});
-          cost[p.tile].push_back(double(ktot) + 8.0);
+          T[shape].push_back(GemmTile{static_cast<int32_t>(pi), b, m0, n0});
+          cost[shape].push_back(double(ktot) + 8.0);
         }
   }
   // longest first (LPT) for the persistent round-robin
   for (int t = 0; t < kNumTileShapes; ++t) {
-    std::vector<size_t> ord(tiles[t].size());
+    std::vector<size_t> ord(T[t].size());
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[t][a] > cost[t][b]; });
     std::vector<GemmTile> sorted;
     sorted.reserve(ord.size());
-    for (auto o : ord) sorted.push_back(tiles[t][o]);
-    tiles[t].swap(sorted);
+    for (auto o : ord) sorted.push_back(T[t][o]);
+    T[t].swap(sorted);
   }
-  uploaded = false;
+  built[mode] = true;
+  tiles_uploaded[mode] = false;
 }
 
 void GemmPlan::upload(hipStream_t s) {
-  d_probs.ensure(std::max<size_t>(probs.size(), 1));
-  d_probs.upload(probs.data(), probs.size(), s);
-  d_segs.ensure(std::max<size_t>(segs.size(), 1));
-  d_segs.upload(segs.data(), segs.size(), s);
-  for (int t = 0; t < kNumTileShapes; ++t) {
-    d_tiles[t].ensure(std::max<size_t>(tiles[t].size(), 1));
-    d_tiles[t].upload(tiles[t].data(), tiles[t].size(), s);
+  if (!uploaded) {
+    d_probs.ensure(std::max<size_t>(probs.size(), 1));
+    d_probs.upload(probs.data(), probs.size(), s);
+    d_segs.ensure(std::max<size_t>(segs.size(), 1));
+    d_segs.upload(segs.data(), segs.size(), s);
+    uploaded = true;
   }
-  uploaded = true;
+  for (int m = 0; m < kTileModes; ++m) {
+    if (!built[m] || tiles_uploaded[m]) continue;
+    for (int t = 0; t < kNumTileShapes; ++t) {
+      d_tiles[m][t].ensure(std::max<size_t>(tiles[m][t].size(), 1));
+      d_tiles[m][t].upload(tiles[m][t].data(), tiles[m][t].size(), s);
+    }
+    tiles_uploaded[m] = true;
+  }
 }
 
 void GemmPlan::clear() {
   probs.clear();
   segs.clear();
-  for (auto& t : tiles) t.clear();
+  for (int m = 0; m < kTileModes; ++m) {
+    for (auto& t : tiles[m]) t.clear();
+    built[m] = false;
+    tiles_uploaded[m] = false;
+  }
   flops = 0.0;
   uploaded = false;
+  tuned[0] = tuned[1] = false;
+}
+
+uint64_t GemmPlan::signature(int precision) const {
+  auto lg = [](int64_t v) { int b = 0; while ((int64_t(1) << b) < v) ++b; return b; };
+  std::map<std::array<int, 4>, double> hist;
+  double tot = 0.0;
+  for (const GemmProb& p : probs) {
+    int64_t k = 0;
+    for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) k += segs[s].K;
+    const double f = double(p.M) * double(p.N) * double(k) * double(p.batch);
+    hist[{lg(p.M), lg(p.N), lg(std::max<int64_t>(k, 1)), p.batch > 1 ? 1 : 0}] += f;
+    tot += f;
+  }
+  uint64_t h = 1469598103934665603ull ^ static_cast<uint64_t>(precision);
+  auto mix = [&h](uint64_t v) { h ^= v; h *= 1099511628211ull; };
+  for (auto& kv : hist) {
+    const int q = tot > 0 ? static_cast<int>(std::lround(20.0 * kv.second / tot)) : 0;
+    if (!q) continue;
+    for (int v : kv.first) mix(static_cast<uint64_t>(v));
+    mix(static_cast<uint64_t>(q));
+  }
+  return h;
+}
+
+namespace {
+
+std::mutex g_tune_mu;
+std::unordered_map<uint64_t, LaunchCfg> g_tune_cache;
+
+void apply_choice(GemmPlan& plan, int precision, const LaunchCfg& c, hipStream_t s) {
+  plan.cfg[precision] = c;
+  if (!plan.built[c.tile_mode]) plan.build_tiles_mode(c.tile_mode);
+  plan.upload(s);
+}
+
+}  // namespace
+
+bool autotune(GemmPlan& plan, int precision, hipStream_t s, const GemmBases* bases, double min_flops) {
+  if (plan.tuned[precision]) return true;
+  const char* e = std::getenv("CAM_AUTOTUNE");
+  if ((e && *e == '0') || plan.flops < min_flops) return false;
+  for (const GemmProb& p : plan.probs)
+    CAM_REQUIRE(!p.beta, "autotune: a plan that accumulates into its output cannot be re-run");
+  const uint64_t key = plan.signature(precision);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      apply_choice(plan, precision, it->second, s);
+      plan.tuned[precision] = true;
+      return true;
+    }
+  }
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  auto time_it = [&](const LaunchCfg& c) {
+    apply_choice(plan, precision, c, s);
+    launch_grouped_gemm(plan, precision, s, bases);   // warm
+    float best = 1e30f;
+    for (int r = 0; r < 2; ++r) {
+      HIP_CHECK(hipEventRecord(ev0, s));
+      launch_grouped_gemm(plan, precision, s, bases);
+      HIP_CHECK(hipEventRecord(ev1, s));
+      HIP_CHECK(hipEventSynchronize(ev1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  // coordinate descent: product form, then tile policy, then grid size
+  LaunchCfg cur = plan.cfg[precision];
+  float tcur = time_it(cur);
+  auto try_axis = [&](int LaunchCfg::*field, std::initializer_list<int> vals) {
+    for (int v : vals) {
+      if (cur.*field == v) continue;
+      LaunchCfg c = cur;
+      c.*field = v;
+      const float t = time_it(c);
+      if (t < tcur) { tcur = t; cur = c; }
+    }
+  };
+  try_axis(&LaunchCfg::algo, {kAlgo3M, kAlgo4M});
+  try_axis(&LaunchCfg::tile_mode, {1, 0, 2});
+  try_axis(&LaunchCfg::wg_per_cu, {4, 2, 8});
+  apply_choice(plan, precision, cur, s);
+  // leave the output of the chosen configuration (every candidate computes
+  // the same values; this keeps the last write consistent with the choice)
+  launch_grouped_gemm(plan, precision, s, bases);
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = cur;
+  }
+  plan.tuned[precision] = true;
+  return true;
 }
 
 void launch_grouped_gemm(GemmPlan& plan, int precision, hipStream_t s, const GemmBases* bases,
                          int grid_limit) {
-  if (!plan.uploaded) plan.upload(s);
+  const LaunchCfg& c = plan.cfg[precision];
+  if (!plan.built[c.tile_mode]) plan.build_tiles_mode(c.tile_mode);
+  if (!plan.uploaded || !plan.tiles_uploaded[c.tile_mode]) plan.upload(s);
   GemmBases b{};
   if (bases) b = *bases;
   if (precision == 0) {
-    launch_shape<double, 64, 64>(plan, 0, s, grid_limit, b);
-    launch_shape<double, 32, 64>(plan, 1, s, grid_limit, b);
-    launch_shape<double, 64, 32>(plan, 2, s, grid_limit, b);
+    launch_shape<double, 64, 64>(plan, c, 0, s, grid_limit, b);
+    launch_shape<double, 32, 64>(plan, c, 1, s, grid_limit, b);
+    launch_shape<double, 64, 32>(plan, c, 2, s, grid_limit, b);
   } else {
-    launch_shape<float, 64, 64>(plan, 0, s, grid_limit, b);
-    launch_shape<float, 32, 64>(plan, 1, s, grid_limit, b);
-    launch_shape<float, 64, 32>(plan, 2, s, grid_limit, b);
+    launch_shape<float, 64, 64>(plan, c, 0, s, grid_limit, b);
+    launch_shape<float, 32, 64>(plan, c, 1, s, grid_limit, b);
+    launch_shape<float, 64, 32>(plan, c, 2, s, grid_limit, b);
   }
 }
 

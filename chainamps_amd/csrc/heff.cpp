@@ -189,9 +189,8 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     segD[pit->second].push_back(SegR{k, e});
   }
 
-  T1.alloc(std::max<int64_t>(t1n, 1));
+  T1.alloc(std::max<int64_t>(std::max(t1n, t3n), 1));
   T2.alloc(std::max<int64_t>(t2n, 1));
-  T3.alloc(std::max<int64_t>(t3n, 1));
 
   // ---- plans: operands addressed as (space, element offset), so the same
   // plan runs on the FP64 tensors and on their FP32 copies ----------------
@@ -265,14 +264,14 @@ GemmBases Heff::bases(int precision, const void* in, void* out) const {
     b.p[kSpR] = R->data.get();
     b.p[kSpT1] = T1.get();
     b.p[kSpT2] = T2.get();
-    b.p[kSpT3] = T3.get();
+    b.p[kSpT3] = T1.get();
   } else {
     b.p[kSpL] = L32.get();
     b.p[kSpW] = W32.get();
     b.p[kSpR] = R32.get();
-    b.p[kSpT1] = T1f.get();
-    b.p[kSpT2] = T2f.get();
-    b.p[kSpT3] = T3f.get();
+    b.p[kSpT1] = T1.get();   // read as float2: offsets count FP32 elements
+    b.p[kSpT2] = T2.get();
+    b.p[kSpT3] = T1.get();
   }
   b.p[kSpIn] = in;
   b.p[kSpOut] = out;
@@ -288,27 +287,61 @@ void Heff::prepare_fp32() {
   launch_to_fp32(L->data.get(), L32.get(), L->total, s);
   launch_to_fp32(W->data.get(), W32.get(), W->total, s);
   launch_to_fp32(R->data.get(), R32.get(), R->total, s);
-  T1f.alloc(std::max<int64_t>(t1n, 1));
-  T2f.alloc(std::max<int64_t>(t2n, 1));
-  T3f.alloc(std::max<int64_t>(t3n, 1));
   vinf.alloc(std::max<int64_t>(n, 1));
   voutf.alloc(std::max<int64_t>(n, 1));
   HIP_CHECK(hipStreamSynchronize(s));
   fp32_ready = true;
 }
 
+namespace {
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
+// The first matvec at a precision (outside graph capture) autotunes every
+// stage plan on its real operands: stage i is re-run while stage i-1's
+// output is already in place, and the last run leaves the correct result.
+void Heff::run_stages(int precision, const GemmBases& b) {
+  hipStream_t s = ctx->stream;
+  const bool tune = !tuned(precision) && !stream_capturing(s);
+  for (int i = 0; i < 4; ++i) {
+    if (tune) autotune(st[i], precision, s, &b);
+    launch_grouped_gemm(st[i], precision, s, &b);
+  }
+}
+
+bool Heff::tuned(int precision) const {
+  for (int i = 0; i < 4; ++i)
+    if (!st[i].tuned[precision] && st[i].flops >= 2e10) return false;
+  return true;
+}
+
+void Heff::set_launch(int precision, const LaunchCfg& c) {
+  for (int i = 0; i < 4; ++i) {
+    st[i].cfg[precision] = c;
+    st[i].tuned[precision] = true;
+    if (!st[i].built[c.tile_mode]) st[i].build_tiles_mode(c.tile_mode);
+    st[i].upload(ctx->stream);
+  }
+}
+
 void Heff::apply_blocked(const cplx* in, cplx* out) {
   hipStream_t s = ctx->stream;
-  const GemmBases b = bases(0, in, out);
-  for (int i = 0; i < 4; ++i) launch_grouped_gemm(st[i], 0, s, &b);
+  run_stages(0, bases(0, in, out));
   for (int pi : untouched) launch_zero(out + pblocks[pi].off, pblocks[pi].size * sizeof(cplx), s);
+}
+
+void Heff::forward_stages(const cplx* in, int nstages) {
+  const GemmBases b = bases(0, in, nullptr);
+  for (int i = 0; i < std::min(nstages, 3); ++i) launch_grouped_gemm(st[i], 0, ctx->stream, &b);
 }
 
 void Heff::apply_blocked32(const float2* in, float2* out) {
   prepare_fp32();
   hipStream_t s = ctx->stream;
-  const GemmBases b = bases(1, in, out);
-  for (int i = 0; i < 4; ++i) launch_grouped_gemm(st[i], 1, s, &b);
+  run_stages(1, bases(1, in, out));
   for (int pi : untouched) launch_zero(out + pblocks[pi].off, pblocks[pi].size * sizeof(float2), s);
 }
 

@@ -55,8 +55,11 @@ struct Heff {
   std::map<std::array<int, 5>, TB> t1, t2, t3;
   int64_t t1n = 0, t2n = 0, t3n = 0;
 
-  DevBuf<cplx> T1, T2, T3, vin, vout;          // FP64
-  DevBuf<float2> L32, W32, R32, T1f, T2f, T3f, vinf, voutf;  // FP32
+  // Two intermediate buffers: stage C writes T3 over T1 (stage D reads only
+  // T3), and an FP32 matvec uses the same bytes as float2 (half the size),
+  // so a matvec needs 2 x max(T) x 16 bytes whatever the precision.
+  DevBuf<cplx> T1, T2, vin, vout;              // T1 also holds T3
+  DevBuf<float2> L32, W32, R32, vinf, voutf;   // FP32 operands
   bool fp32_ready = false;
   GemmPlan st[4];
   std::vector<int> untouched;   // psi blocks no stage-D problem writes
@@ -74,6 +77,13 @@ struct Heff {
   GemmBases bases(int precision, const void* in, void* out) const;
   // device block-layout vectors
   void apply_blocked(const cplx* in, cplx* out);
+  // the four stage launches (autotuning them on first use outside capture)
+  void run_stages(int precision, const GemmBases& b);
+  bool tuned(int precision) const;
+  // fixes the launch configuration of every stage (no autotuning)
+  void set_launch(int precision, const LaunchCfg& c);
+  // FP64 stages A..(nstages-1) only: leaves T1 (and T2) of `in` for the adjoint
+  void forward_stages(const cplx* in, int nstages);
   void apply_blocked32(const float2* in, float2* out);
   // precision: 0 fp64, 1 fp32 (input/output FP64 block layout)
   void apply(const cplx* in, cplx* out, int precision);

@@ -92,7 +92,7 @@ struct Lanczos {
   int64_t n;
   int m;                      // Krylov dimension
   DevBuf<cplx> V;             // (m+1) x n
-  DevBuf<cplx> w, x, y;       // work
+  DevBuf<cplx> w;             // work (the Ritz vector lives in the caller's buffer)
   DevBuf<cplx> coef, partial;
   DevBuf<cplx> C1, C2, Nrm, Dec;   // per-step CGS2 coefficients, norms, decisions
   DevBuf<double> dD, dE, dC;
@@ -106,8 +106,6 @@ struct Lanczos {
     CAM_REQUIRE(m + 1 <= 64, "ncv too large (max 63)");
     V.alloc(static_cast<size_t>(n) * (m + 1));
     w.alloc(n);
-    x.alloc(n);
-    y.alloc(n);
     coef.alloc(64);
     partial.alloc(static_cast<size_t>(multi_dot_blocks(n)) * 64);
     C1.alloc(64 * 64);
@@ -158,6 +156,8 @@ struct Lanczos {
   // alpha_1.., beta_1.. up to the first breakdown
   bool run_cycle(int prec, std::vector<double>& alpha, std::vector<double>& beta, int& k) {
     if (m > 1) {
+      // launch configurations are fixed before they are captured
+      if (!H.tuned(prec)) H.apply(col(1), w.get(), prec);
       hipGraphExec_t& ge = cycle_exec[prec];
       if (!ge && use_graphs) {
         HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -352,17 +352,16 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
   // start vector (SparseVectorWithRestriction -> DumbExtractWithZeros); a
   // zero guess takes the info=-9 path: random start
   if (have_start) {
-    launch_copy(xout, L.x.get(), n, s);
-    have_start = L.norm(L.x.get()) > 0.0;
+    have_start = L.norm(xout) > 0.0;
   }
   if (!have_start) {
     hipLaunchKernelGGL(random_fill_kernel, dim3(1024), dim3(256), 0, s,
-                       reinterpret_cast<double2*>(L.x.get()), n, 0x1234ull);
+                       reinterpret_cast<double2*>(xout), n, 0x1234ull);
     HIP_CHECK(hipGetLastError());
   }
   {
-    const double nx = L.norm(L.x.get());
-    launch_scale(cplx{1.0 / nx, 0.0}, L.x.get(), n, s);
+    const double nx = L.norm(xout);
+    launch_scale(cplx{1.0 / nx, 0.0}, xout, n, s);
   }
   // ARPACK's maxiter: 50 implicit restarts converged, 2 for growth steps
   // (arpack_interface.cpp:28); explicit restarts need more when converging
@@ -372,7 +371,7 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
   bool use32 = mixed;
   while (true) {
     const int m = L.m;
-    launch_copy(L.x.get(), L.col(0), n, s);
+    launch_copy(xout, L.col(0), n, s);
     // step 0 in FP64: exact Rayleigh quotient and residual of x
     L.matvec(L.col(0), L.w.get(), 0);
     auto c0 = L.orthogonalize(1, L.w.get());
@@ -401,9 +400,9 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
     std::vector<std::complex<double>> cs(k);
     for (int j = 0; j < k; ++j) cs[j] = svec[j];
     HIP_CHECK(hipMemcpyAsync(L.coef.get(), cs.data(), k * sizeof(cplx), hipMemcpyHostToDevice, s));
-    launch_combine(L.V.get(), n, k, L.coef.get(), L.x.get(), n, s);
-    const double nx = L.norm(L.x.get());
-    launch_scale(cplx{1.0 / nx, 0.0}, L.x.get(), n, s);
+    launch_combine(L.V.get(), n, k, L.coef.get(), xout, n, s);
+    const double nx = L.norm(xout);
+    launch_scale(cplx{1.0 / nx, 0.0}, xout, n, s);
     ++restarts;
     // FP64 runs stop on ARPACK's Ritz estimate; mixed runs re-check the new
     // vector with an exact FP64 matvec at the top of the loop
@@ -419,7 +418,6 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
     info[2] = restarts;
     info[3] = resid;
   }
-  launch_copy(L.x.get(), xout, n, s);
   gauge_fix(xout, n, s);
   HIP_CHECK(hipStreamSynchronize(s));
   return theta;

@@ -60,7 +60,10 @@ void heff_vjp(Heff& H, const cplx* psi_ref, cplx* dpsi_ref, cplx* dW_out, int64_
   DevBuf<cplx> r(n), x(n), y(n);
   r.upload(psi_ref, n, s);
   launch_scatter_map(r.get(), x.get(), H.perm.get(), n, s);
-  H.apply(x.get(), y.get(), 0);            // leaves T1, T2, T3 of this psi
+  H.apply(x.get(), y.get(), 0);
+  // the matvec's stage C wrote T3 over T1: recompute T1 and T2 of this psi
+  // (the environments are recomputed, not stored per Lanczos step)
+  H.forward_stages(x.get(), 2);
   DevBuf<cplx> partial(static_cast<size_t>(multi_dot_blocks(n)) * 64), coef(64);
   auto dot = [&](const cplx* a, const cplx* b) {
     launch_multi_dot(a, n, 1, b, n, partial.get(), coef.get(), s);

@@ -84,7 +84,16 @@ def coupled_chain_mpo(ctx, d=64, seed=0, Jperp=0.1):
     return tensor_from_dense(ctx, rules, idx, 2, T), rules, phys, bq
 
 
-def random_left_block(ctx, rules, aq, bq, rng):
+def _hermitian_channels(T, mask, rows, cols):
+    """Make every MPO channel b of a random environment a Hermitian
+    (a' x a) matrix, so that H_eff is Hermitian for a Hermitian MPO whose
+    channels are self-conjugate (the Z2 coupled chain)."""
+    M = np.moveaxis(T.reshape(rows), [0, 1, 2], [1, 0, 2])      # (b, a', a)
+    M = 0.5 * (M + np.conj(np.swapaxes(M, 1, 2)))
+    return np.where(mask, np.moveaxis(M, [0, 1, 2], [1, 0, 2]).reshape(cols), 0)
+
+
+def random_left_block(ctx, rules, aq, bq, rng, hermitian=False):
     """[dummy(out), a'(in), dummy(in), b(out), dummy(in), a(out)]."""
     z = np.zeros((1, len(rules)), np.int64)
     idx = [IndexDesc(False, False, z), IndexDesc(True, False, aq), IndexDesc(True, False, z),
@@ -92,10 +101,12 @@ def random_left_block(ctx, rules, aq, bq, rng):
     dims = [1, len(aq), 1, len(bq), 1, len(aq)]
     mask = _flux_mask(rules, idx, dims)
     T = np.where(mask, rng.standard_normal(dims) + 1j * rng.standard_normal(dims), 0)
+    if hermitian:
+        T = _hermitian_channels(T, mask, (len(aq), len(bq), len(aq)), dims)
     return tensor_from_dense(ctx, rules, idx, 3, T)
 
 
-def random_right_block(ctx, rules, aq, bq, rng):
+def random_right_block(ctx, rules, aq, bq, rng, hermitian=False):
     """[a'(out), dummy(in), b(in), dummy(out), a(in), dummy(out)]."""
     z = np.zeros((1, len(rules)), np.int64)
     idx = [IndexDesc(False, False, aq), IndexDesc(True, False, z), IndexDesc(True, False, bq),
@@ -103,16 +114,20 @@ def random_right_block(ctx, rules, aq, bq, rng):
     dims = [len(aq), 1, len(bq), 1, len(aq), 1]
     mask = _flux_mask(rules, idx, dims)
     T = np.where(mask, rng.standard_normal(dims) + 1j * rng.standard_normal(dims), 0)
+    if hermitian:
+        T = _hermitian_channels(T, mask, (len(aq), len(bq), len(aq)), dims)
     return tensor_from_dense(ctx, rules, idx, 2, T)
 
 
-def coupled_chain_heff_inputs(ctx, chi=500, d=64, seed=0, Jperp=0.1):
-    """Z2 environments with two equal parity sectors of chi/2 states."""
+def coupled_chain_heff_inputs(ctx, chi=500, d=64, seed=0, Jperp=0.1, hermitian=False):
+    """Z2 environments with two equal parity sectors of chi/2 states;
+    hermitian=True gives Hermitian environment channels (an H_eff the
+    eigensolver can be timed on)."""
     rng = np.random.default_rng(seed + 1)
     W, rules, phys, bq = coupled_chain_mpo(ctx, d, seed, Jperp)
     aq = (np.arange(chi) % 2).reshape(-1, 1)
-    L = random_left_block(ctx, rules, aq, bq, rng)
-    R = random_right_block(ctx, rules, aq, bq, rng)
+    L = random_left_block(ctx, rules, aq, bq, rng, hermitian)
+    R = random_right_block(ctx, rules, aq, bq, rng, hermitian)
     return L, W, R
 
 

@@ -122,6 +122,16 @@ int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int preci
  * (L, W, W, R), the average launch duration in ms measured with HIP events
  * on the context stream and the algorithmic flops of that stage. */
 int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, double* stage_flops);
+/* Launch configuration of the four stage GEMMs (no reference counterpart:
+ * the reference's SpGEMM has no tunables).  By default the first matvec at
+ * each precision autotunes them (timed candidates, cached per block-size
+ * histogram; CAM_AUTOTUNE=0 disables).  set fixes it for every stage:
+ * algo 0 = four real MFMAs per complex product, 1 = three (Gauss);
+ * tile_mode 0 = 64x64 tiles, 1 = per-block least padding, 2 = 32x64/64x32
+ * only; wg_per_cu = resident workgroups per CU of the persistent grid.
+ * get returns {algo, tile_mode, wg_per_cu} per stage (12 ints). */
+int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu);
+int cam_heff_get_launch(const cam_heff* h, int precision, int* cfg);
 /* Reference order <-> device block layout (K3 gather/scatter). */
 int cam_heff_to_blocked(cam_heff* h, const double* in_dev, double* out_dev);
 int cam_heff_from_blocked(cam_heff* h, const double* in_dev, double* out_dev);
@@ -141,6 +151,12 @@ int cam_heff_from_blocked(cam_heff* h, const double* in_dev, double* out_dev);
  * restarts, residual_norm}. */
 int cam_heff_lowest(cam_heff* h, const double* guess, int converge, double tol,
                     int ncv, int mixed, double* eval, double* evec, double* info);
+/* The same eigensolve on a device vector in the block layout (the form the
+ * iDMRG driver keeps between steps; no host round trip of the n-vector):
+ * x holds the start vector when have_start (a zero vector means a random
+ * start) and receives the eigenvector. */
+int cam_heff_lowest_blocked(cam_heff* h, double* x, int have_start, int converge, double tol,
+                            int ncv, int mixed, double* eval, double* info);
 
 /* ---- blockwise SVD truncation -------------------------------------------
  * MPX_matrix::SVD -> SparseMatrix::SVD (MPX_matrix.cpp:157-192,

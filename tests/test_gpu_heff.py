@@ -76,6 +76,26 @@ def test_heff_matvec_parity(ctx, case):
         assert rel_err(out32, ref) <= 2e-5, (case, trial)
 
 
+@pytest.mark.parametrize("algo,tile_mode,wg", [(0, 1, 4), (1, 1, 4), (1, 0, 2), (0, 2, 8), (1, 2, 1)])
+def test_heff_launch_configurations(ctx, algo, tile_mode, wg):
+    """Every launch configuration the autotuner can pick (four or three real
+    MFMAs per complex product, each tile policy, grid sizes) gives the same
+    matvec to the same bar."""
+    print(f"case heff_launch algo={algo} tile_mode={tile_mode} wg={wg}", flush=True)
+    import chainamps_amd as cam
+    rules, LB, W, RB = CASES["random_u1_w6"]()
+    comp = dmrg.TwoVertexComponents(LB, W, RB)
+    H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
+    for prec in (cam.CAM_FP64, cam.CAM_FP32):
+        H.set_launch(prec, algo, tile_mode, wg)
+        assert H.get_launch(prec) == [(algo, tile_mode, wg)] * 4
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal(comp.n) + 1j * rng.standard_normal(comp.n)
+    ref = comp.matvec(v)
+    assert rel_err(H.apply(v), ref) <= 1e-12
+    assert rel_err(H.apply(v, cam.CAM_FP32), ref) <= 2e-5
+
+
 def test_heff_hermitian_and_linear(ctx):
     print("case heff_hermitian", flush=True)
     import chainamps_amd as cam
