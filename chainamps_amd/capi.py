@@ -478,3 +478,59 @@ class IDMRG:
             self.close()
         except Exception:
             pass
+
+
+_SIGS_ITEBD = {
+    "cam_itebd_create": [_p, _p, _p, _p, _dp, C.c_int64, _dp, C.c_int64, C.c_double, C.c_int, C.POINTER(_p)],
+    "cam_itebd_evolve": [_p, C.c_int64, C.c_int64, C.c_double, _dp],
+    "cam_itebd_lambda": [_p, C.c_int, _dp, C.c_int64, _i64p],
+    "cam_itebd_tensor": [_p, C.c_int, C.POINTER(_p)],
+    "cam_itebd_destroy": [_p],
+}
+_SIGS.update(_SIGS_ITEBD)
+
+ITEBD_FIELDS = ("steps", "time", "truncation", "entropy", "chi", "seconds", "gate_flops")
+
+
+class ITEBD:
+    """Device iTEBD (TrotterDecomposition + iTEBD::evolve, orders 1 and 2)."""
+
+    def __init__(self, W: Tensor, A0: Tensor, A1: Tensor, lambda0, lambda1, dt, order=2):
+        self.ctx = W.ctx
+        l0 = np.ascontiguousarray(lambda0, dtype=np.float64)
+        l1 = np.ascontiguousarray(lambda1, dtype=np.float64)
+        h = _p()
+        _check(_LIB.cam_itebd_create(W.ctx.h, W.h, A0.h, A1.h, _ptr(l0, C.c_double), len(l0),
+                                     _ptr(l1, C.c_double), len(l1), float(dt), int(order), C.byref(h)))
+        self.h = h
+
+    def evolve(self, steps, D=0, minS=0.0):
+        rec = np.zeros(7)
+        _check(_LIB.cam_itebd_evolve(self.h, int(steps), int(D), float(minS), _ptr(rec, C.c_double)))
+        d = dict(zip(ITEBD_FIELDS, rec.tolist()))
+        d["steps"] = int(d["steps"])
+        d["chi"] = int(d["chi"])
+        return d
+
+    def lam(self, k):
+        n = C.c_int64()
+        _check(_LIB.cam_itebd_lambda(self.h, int(k), None, 0, C.byref(n)))
+        v = np.zeros(max(n.value, 1))
+        _check(_LIB.cam_itebd_lambda(self.h, int(k), _ptr(v, C.c_double), len(v), C.byref(n)))
+        return v[:n.value]
+
+    def tensor(self, k) -> Tensor:
+        h = _p()
+        _check(_LIB.cam_itebd_tensor(self.h, int(k), C.byref(h)))
+        return Tensor(self.ctx, h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_itebd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,6 +8,7 @@
 #include "heff.hpp"
 #include "solvers.hpp"
 #include "idmrg.hpp"
+#include "itebd.hpp"
 #include <memory>
 
 using namespace cam;
@@ -420,6 +421,11 @@ struct cam_idmrg_s {
   cam_ctx* owner;
 };
 
+struct cam_itebd_s {
+  std::unique_ptr<cam::ITEBD> d;
+  cam_ctx* owner;
+};
+
 extern "C" {
 
 int cam_idmrg_create(cam_ctx* c, const cam_tensor* W, const int16_t* target, int converge, int mixed, int ncv,
@@ -456,6 +462,61 @@ int cam_idmrg_values(const cam_idmrg* d, double* values, int64_t cap, int64_t* n
 int cam_idmrg_destroy(cam_idmrg* d) {
   CAM_API_BEGIN
   delete d;
+  CAM_API_END
+}
+
+int cam_itebd_create(cam_ctx* c, const cam_tensor* W, const cam_tensor* A0, const cam_tensor* A1,
+                     const double* lambda0, int64_t n0, const double* lambda1, int64_t n1, double dt, int order,
+                     cam_itebd** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  CAM_REQUIRE(lambda0 && lambda1 && n0 > 0 && n1 > 0, "iTEBD: bond weights missing");
+  auto* r = new cam_itebd;
+  r->owner = c;
+  try {
+    r->d.reset(new cam::ITEBD(c->ctx, *W->t, A0->t, A1->t, std::vector<double>(lambda0, lambda0 + n0),
+                              std::vector<double>(lambda1, lambda1 + n1), dt, order));
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  *out = r;
+  CAM_API_END
+}
+
+int cam_itebd_evolve(cam_itebd* it, int64_t nsteps, int64_t D, double minS, double* record) {
+  CAM_API_BEGIN
+  set_device(it->owner);
+  const cam::ITEBDRecord r = it->d->evolve(nsteps, D, minS);
+  const double v[7] = {double(r.steps), r.time, r.truncation, r.entropy, double(r.chi), r.seconds, r.gate_flops};
+  if (record) std::copy(v, v + 7, record);
+  CAM_API_END
+}
+
+int cam_itebd_lambda(const cam_itebd* it, int k, double* values, int64_t cap, int64_t* n) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(k == 0 || k == 1, "iTEBD: bond index must be 0 or 1");
+  const auto& v = it->d->lam[k];
+  if (n) *n = static_cast<int64_t>(v.size());
+  CAM_REQUIRE(static_cast<int64_t>(v.size()) <= cap || !values, "capacity too small");
+  if (values) std::copy(v.begin(), v.end(), values);
+  CAM_API_END
+}
+
+int cam_itebd_tensor(const cam_itebd* it, int k, cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(it->owner);
+  CAM_REQUIRE(k == 0 || k == 1, "iTEBD: tensor index must be 0 or 1");
+  const cam::BSTensor& A = *it->d->A[k];
+  std::vector<int> ident(A.nidx());
+  for (int i = 0; i < A.nidx(); ++i) ident[i] = i;
+  *out = new cam_tensor{reorder(it->owner->ctx, A, false, ident, A.numrows), it->owner};
+  CAM_API_END
+}
+
+int cam_itebd_destroy(cam_itebd* it) {
+  CAM_API_BEGIN
+  delete it;
   CAM_API_END
 }
 

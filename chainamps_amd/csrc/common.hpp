@@ -68,6 +68,13 @@ class DevBuf {
   size_t n_ = 0;
 };
 
+// Non-owning view into a workspace DevBuf (same get() interface).
+template <typename T>
+struct DevView {
+  T* p = nullptr;
+  T* get() const { return p; }
+};
+
 struct cplx {
   double re, im;
 };

@@ -258,6 +258,7 @@ TensorPtr contract(Ctx& ctx, const BSTensor& A, bool conjA, const BSTensor& B, b
   plan.choose_tiles(1);
   plan.build_tiles();
   launch_grouped_gemm(plan, 0, ctx.stream);
+  ctx.contract_flops += plan.flops;
   HIP_CHECK(hipStreamSynchronize(ctx.stream));
   return out;
 }

@@ -17,6 +17,12 @@ struct Ctx {
   DevBuf<cplx> scratchA, scratchB;
   DevBuf<PermJob> perm_jobs;
   DevBuf<int64_t> perm_chunks;
+  double contract_flops = 0.0;   // algorithmic 8mnk of every contract() so far
+  // Grow-only workspaces reused across calls on this context (one stream,
+  // one host thread): the Lanczos basis and the two H_eff intermediates.
+  // Freeing and re-allocating ~10^11-byte buffers between eigensolves costs
+  // seconds near the memory limit, so they are kept until the context ends.
+  DevBuf<cplx> ws_krylov, ws_t1, ws_t2;
   explicit Ctx(int dev);
   ~Ctx();
   void* blas_handle();

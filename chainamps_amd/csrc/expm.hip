@@ -12,7 +12,7 @@
 //     Hermitian decomposition SparseMatrix::HED (sparse_interface.cpp:
 //     1414-1441): per group of columns with equal outgoing charge, the
 //     full square block (zero eigenvalues kept) is diagonalised (rocSOLVER
-//     zheevd), eigenvector entries |v| <= eps/rows are dropped, and
+//     zheev), eigenvector entries |v| <= eps/rows are dropped, and
 //     G = V diag(exp(-i tau lambda)) V^dagger (rocBLAS zgemm).  The result
 //     keeps the first four indices of the bond Hamiltonian.
 #include <rocblas/rocblas.h>
@@ -186,9 +186,9 @@ TensorPtr bond_evolution_operator(Ctx& ctx, const BSTensor& H, double timestep) 
   for (int g = 0; g < ng; ++g) {
     const int64_t m = gsize[g];
     auto* Ag = reinterpret_cast<rocblas_double_complex*>(A.get() + goff[g]);
-    rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(m), Ag,
+    rocblas_status st = rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(m), Ag,
                                          static_cast<rocblas_int>(m), lam.get(), off.get(), dinfo.get());
-    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheevd failed");
+    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheev failed");
     int info = 0;
     HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));

@@ -30,6 +30,7 @@ namespace cam {
 namespace {
 
 constexpr int KC = 16;     // K chunk staged in LDS
+constexpr int kXcds = 8;   // MI355X accelerator complex dies (one L2 each)
 constexpr int PADL = 4;    // LDS row padding (elements)
 
 // global-address-space access (global_load / global_store rather than
@@ -104,7 +105,12 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
   const int wm = wave & 1, wn = wave >> 1;
   const int fr = lane & 15, fk = lane >> 4;
 
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  // XCD-aware order: workgroup b runs on XCD b % 8, so consecutive tiles
+  // (the M-tiles of one output column block, which read the same B columns)
+  // are given to workgroups of the same XCD and share its L2
+  const int G = gridDim.x;
+  const int v0 = (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+  for (int t = v0; t < ntiles; t += G) {
     const GemmTile tl = tiles[t];
     const GemmProb P = probs[tl.prob];
     const int M = P.M, N = P.N;

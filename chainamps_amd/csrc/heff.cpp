@@ -189,8 +189,7 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     segD[pit->second].push_back(SegR{k, e});
   }
 
-  T1.alloc(std::max<int64_t>(std::max(t1n, t3n), 1));
-  T2.alloc(std::max<int64_t>(t2n, 1));
+  bind_workspace();
 
   // ---- plans: operands addressed as (space, element offset), so the same
   // plan runs on the FP64 tensors and on their FP32 copies ----------------
@@ -256,7 +255,16 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
-GemmBases Heff::bases(int precision, const void* in, void* out) const {
+void Heff::bind_workspace() {
+  ctx->ws_t1.ensure(std::max<int64_t>(std::max(t1n, t3n), 1));
+  ctx->ws_t2.ensure(std::max<int64_t>(t2n, 1));
+  T1.p = ctx->ws_t1.get();
+  T2.p = ctx->ws_t2.get();
+}
+
+GemmBases Heff::bases(int precision, const void* in, void* out) {
+  // another H_eff on this context may have grown the shared workspaces
+  bind_workspace();
   GemmBases b{};
   if (precision == 0) {
     b.p[kSpL] = L->data.get();

@@ -58,7 +58,8 @@ struct Heff {
   // Two intermediate buffers: stage C writes T3 over T1 (stage D reads only
   // T3), and an FP32 matvec uses the same bytes as float2 (half the size),
   // so a matvec needs 2 x max(T) x 16 bytes whatever the precision.
-  DevBuf<cplx> T1, T2, vin, vout;              // T1 also holds T3
+  DevView<cplx> T1, T2;                        // context workspaces; T1 also holds T3
+  DevBuf<cplx> vin, vout;
   DevBuf<float2> L32, W32, R32, vinf, voutf;   // FP32 operands
   bool fp32_ready = false;
   GemmPlan st[4];
@@ -74,7 +75,9 @@ struct Heff {
   void build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_);
   void build_perm();
   void prepare_fp32();
-  GemmBases bases(int precision, const void* in, void* out) const;
+  // operand bases of a launch (re-binds T1/T2 to the context workspaces)
+  GemmBases bases(int precision, const void* in, void* out);
+  void bind_workspace();
   // device block-layout vectors
   void apply_blocked(const cplx* in, cplx* out);
   // the four stage launches (autotuning them on first use outside capture)

@@ -1,0 +1,90 @@
+// Device-resident iTEBD.
+//
+// TrotterDecomposition (TEBD_routines.cpp:14-51): order 1 uses
+// {G_odd(tau), G_even(tau)}, order 2 {G_odd(tau/2), G_even(tau), G_odd(tau)};
+// order 4 canonises the unit cell between pieces
+// (OrthogonaliseInversionSymmetric), which is not on this path and is
+// rejected.  iTEBD::evolve (TEBD_routines.cpp:120-185) without measurement
+// steps: order 1 applies ops 0, 1 per step; order 2 applies op 0, then
+// (1, 2) for all but the last step, then 1, 0 and swaps the cell back.
+// Each gate is iTEBD::apply_and_decompose (TEBD_routines.cpp:233-266),
+// which leaves the cell swapped: A = {A1', U}, lambda = {new, old lambda0}.
+#include <chrono>
+#include <cmath>
+
+#include "itebd.hpp"
+
+namespace cam {
+
+ITEBD::ITEBD(Ctx& c, const BSTensor& W, TensorPtr A0, TensorPtr A1, std::vector<double> l0,
+             std::vector<double> l1, double time_step, int trotter_order)
+    : ctx(&c), order(trotter_order), dt(time_step) {
+  CAM_REQUIRE(order == 1 || order == 2,
+              "iTEBD: Trotter orders 1 and 2 (order 4 canonises the cell between pieces)");
+  CAM_REQUIRE(A0 && A1 && A0->nidx() == 3 && A1->nidx() == 3, "iTEBD: the unit cell needs two 3-index tensors");
+  CAM_REQUIRE(!l0.empty() && !l1.empty(), "iTEBD: empty bond weights");
+  TensorPtr Hodd = bond_hamiltonian(c, W, true);
+  TensorPtr Heven = bond_hamiltonian(c, W, false);
+  if (order == 1) {
+    ops.push_back(bond_evolution_operator(c, *Hodd, dt));
+    ops.push_back(bond_evolution_operator(c, *Heven, dt));
+  } else {
+    ops.push_back(bond_evolution_operator(c, *Hodd, 0.5 * dt));
+    ops.push_back(bond_evolution_operator(c, *Heven, dt));
+    ops.push_back(bond_evolution_operator(c, *Hodd, dt));
+  }
+  A[0] = std::move(A0);
+  A[1] = std::move(A1);
+  lam[0] = std::move(l0);
+  lam[1] = std::move(l1);
+}
+
+double ITEBD::gate(int op, int64_t D, double minS) {
+  TensorPtr n0, n1;
+  std::vector<double> nl;
+  double tr = 0.0;
+  tebd_gate(*ctx, *ops.at(op), *A[0], *A[1], lam[0], D, minS, n0, n1, nl, tr);
+  A[0] = n0;
+  A[1] = n1;
+  lam[1] = std::move(lam[0]);
+  lam[0] = std::move(nl);
+  truncation = tr;
+  return tr;
+}
+
+ITEBDRecord ITEBD::evolve(int64_t nsteps, int64_t D, double minS) {
+  CAM_REQUIRE(nsteps >= 0, "iTEBD: negative step count");
+  const auto t0 = std::chrono::steady_clock::now();
+  const double f0 = ctx->contract_flops;
+  if (order == 1) {
+    for (int64_t s = 0; s < nsteps; ++s) {
+      gate(0, D, minS);
+      gate(1, D, minS);
+    }
+  } else if (nsteps > 0) {
+    gate(0, D, minS);
+    for (int64_t s = 0; s + 1 < nsteps; ++s) {
+      gate(1, D, minS);
+      gate(2, D, minS);
+    }
+    gate(1, D, minS);
+    gate(0, D, minS);
+    std::swap(A[0], A[1]);   // m_unit.swap(0,1): back to the 12 order
+    std::swap(lam[0], lam[1]);
+  }
+  steps += nsteps;
+  ITEBDRecord r{};
+  r.steps = steps;
+  r.time = dt * static_cast<double>(steps);
+  r.truncation = truncation;
+  double s = 0.0;
+  for (double x : lam[0])
+    if (x != 0.0) s -= x * x * std::log(x * x);
+  r.entropy = s;
+  r.chi = static_cast<int64_t>(lam[0].size());
+  r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  r.gate_flops = ctx->contract_flops - f0;
+  return r;
+}
+
+}  // namespace cam

@@ -1,0 +1,39 @@
+// Device-resident iTEBD time evolution (iTEBD::evolve mirror).
+#pragma once
+#include <vector>
+
+#include "solvers.hpp"
+
+namespace cam {
+
+struct ITEBDRecord {
+  int64_t steps;        // time steps taken so far
+  double time;          // steps * dt
+  double truncation;    // of the last gate
+  double entropy;       // of lambda0 (sum over nonzero -l^2 log l^2)
+  int64_t chi;          // length of lambda0
+  double seconds;       // wall time of this evolve call
+  double gate_flops;    // algorithmic flops of the gate contractions of this call
+};
+
+// TrotterDecomposition (TEBD_routines.cpp:14-85) for orders 1 and 2 and the
+// evolve loop (TEBD_routines.cpp:120-185) without measurement steps; the
+// unit cell {A0, A1, lambda0, lambda1} stays on the device.
+struct ITEBD {
+  Ctx* ctx;
+  int order;
+  double dt;
+  std::vector<TensorPtr> ops;   // BondOperators
+  TensorPtr A[2];
+  std::vector<double> lam[2];
+  int64_t steps = 0;
+  double truncation = 0.0;
+
+  ITEBD(Ctx& c, const BSTensor& W, TensorPtr A0, TensorPtr A1, std::vector<double> l0, std::vector<double> l1,
+        double time_step, int trotter_order);
+  // one apply_and_decompose with the unit-cell swap
+  double gate(int op, int64_t D, double minS);
+  ITEBDRecord evolve(int64_t nsteps, int64_t D, double minS);
+};
+
+}  // namespace cam

@@ -91,8 +91,8 @@ struct Lanczos {
   hipStream_t s;
   int64_t n;
   int m;                      // Krylov dimension
-  DevBuf<cplx> V;             // (m+1) x n
-  DevBuf<cplx> w;             // work (the Ritz vector lives in the caller's buffer)
+  DevView<cplx> V;            // (m+1) x n, in the context's Krylov workspace
+  DevView<cplx> w;            // work (the Ritz vector lives in the caller's buffer)
   DevBuf<cplx> coef, partial;
   DevBuf<cplx> C1, C2, Nrm, Dec;   // per-step CGS2 coefficients, norms, decisions
   DevBuf<double> dD, dE, dC;
@@ -104,8 +104,9 @@ struct Lanczos {
   Lanczos(Heff& h, int ncv) : H(h), s(h.ctx->stream), n(h.n) {
     m = std::max(1, static_cast<int>(std::min<int64_t>(ncv, n)));
     CAM_REQUIRE(m + 1 <= 64, "ncv too large (max 63)");
-    V.alloc(static_cast<size_t>(n) * (m + 1));
-    w.alloc(n);
+    h.ctx->ws_krylov.ensure(static_cast<size_t>(n) * (m + 2));
+    V.p = h.ctx->ws_krylov.get();
+    w.p = V.p + static_cast<size_t>(n) * (m + 1);
     coef.alloc(64);
     partial.alloc(static_cast<size_t>(multi_dot_blocks(n)) * 64);
     C1.alloc(64 * 64);
@@ -446,7 +447,7 @@ __global__ void cut_small_kernel(const double2* __restrict__ x, double2* __restr
 // Dense branch of TwoVertexComponents::HED (DMRG_routines.cpp:850-858): the
 // whole H_eff is formed (column j = H_eff e_j, in the block layout, which is
 // a permutation of the reference's allowed-index order) and diagonalised
-// with rocSOLVER zheevd; the lowest eigenvector is returned with entries
+// with rocSOLVER zheev; the lowest eigenvector is returned with entries
 // below SPARSETOL cut and no phase gauge, as the reference's LAPACK path.
 double dense_lowest(Heff& H, cplx* xout, double* info) {
   const int64_t n = H.n;
@@ -462,19 +463,24 @@ double dense_lowest(Heff& H, cplx* xout, double* info) {
     H.apply(e.get(), A.get() + j * n, 0);
   }
   rocblas_handle h = static_cast<rocblas_handle>(H.ctx->blas_handle());
-  const rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(n),
+  const rocblas_status st = rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(n),
                                              reinterpret_cast<rocblas_double_complex*>(A.get()),
                                              static_cast<rocblas_int>(n), D.get(), off.get(), dinfo.get());
-  CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheevd failed");
+  CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheev failed");
   hipLaunchKernelGGL(cut_small_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const double2*>(A.get()),
                      reinterpret_cast<double2*>(xout), n, std::numeric_limits<double>::epsilon());
   HIP_CHECK(hipGetLastError());
+  DevBuf<cplx> partial(static_cast<size_t>(multi_dot_blocks(n)) * 64), nrm(1);
+  launch_multi_dot(xout, n, 1, xout, n, partial.get(), nrm.get(), s);
   double e0 = 0.0;
   int inf = 0;
+  cplx nx{0.0, 0.0};
   HIP_CHECK(hipMemcpyAsync(&e0, D.get(), sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(&inf, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&nx, nrm.get(), sizeof(cplx), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   CAM_REQUIRE(inf == 0, "dense eigensolver did not converge");
+  CAM_REQUIRE(std::fabs(nx.re - 1.0) < 1e-6, "dense eigensolver returned a vector that is not normalised");
   if (info) {
     info[0] = static_cast<double>(n);
     info[1] = 0.0;

@@ -212,6 +212,28 @@ int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record)
 int cam_idmrg_values(const cam_idmrg* d, double* values, int64_t cap, int64_t* n);
 int cam_idmrg_destroy(cam_idmrg* d);
 
+/* ---- iTEBD time evolution ------------------------------------------------
+ * iTEBD (TEBD_routines.cpp:14-85 TrotterDecomposition, 120-185 evolve,
+ * 233-266 apply_and_decompose) with the unit cell on the device: W the
+ * model MPO, A0/A1 the left-shape cell tensors [s, a_l, a_r], lambda0/1
+ * the bond weights, dt the time step, order 1 or 2 (order 4 needs the
+ * cell canonisation between Trotter pieces and is rejected).  evolve runs
+ * nsteps full time steps with no measurement in between; the second-order
+ * step is odd(dt/2) even(dt) odd(dt) ... even(dt) odd(dt/2) with the odd
+ * full-step operator between steps.  record[7] receives {steps so far,
+ * time, truncation of the last gate, entropy of lambda0, chi, seconds,
+ * gate-contraction flops of the call}. */
+typedef struct cam_itebd_s cam_itebd;
+int cam_itebd_create(cam_ctx* ctx, const cam_tensor* W, const cam_tensor* A0, const cam_tensor* A1,
+                     const double* lambda0, int64_t n0, const double* lambda1, int64_t n1, double dt,
+                     int order, cam_itebd** out);
+int cam_itebd_evolve(cam_itebd* it, int64_t nsteps, int64_t D, double minS, double* record);
+/* bond weights k (0 or 1), capacity cap; *n receives the count */
+int cam_itebd_lambda(const cam_itebd* it, int k, double* values, int64_t cap, int64_t* n);
+/* a copy of cell tensor k (0 or 1) as a new tensor handle */
+int cam_itebd_tensor(const cam_itebd* it, int k, cam_tensor** out);
+int cam_itebd_destroy(cam_itebd* it);
+
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * Vector-Jacobian product of E = <psi|H_eff|psi>/<psi|psi>:
  * dpsi (host, n complex, reference order) = dE/dconj(psi) and

@@ -88,3 +88,52 @@ def product_state_cell(rules, phys, level=0):
         T[level, 0, 0] = 1.0
         return MPX.from_dense_tensor(rules, idx, 2, T)
     return site(z, q1), site(q1, q2), np.array([1.0]), np.array([1.0])
+
+
+def trotter_operators(W: MPX, tau: float, order: int):
+    """TrotterDecomposition (TEBD_routines.cpp:14-51): the bond operators of
+    order 1 (odd tau, even tau) and order 2 (odd tau/2, even tau, odd tau)."""
+    Hodd, Heven = odd_bond_hamiltonian(W), even_bond_hamiltonian(W)
+    if order == 1:
+        return [bond_evolution_operator(Hodd, tau), bond_evolution_operator(Heven, tau)]
+    if order == 2:
+        return [bond_evolution_operator(Hodd, 0.5 * tau), bond_evolution_operator(Heven, tau),
+                bond_evolution_operator(Hodd, tau)]
+    raise ValueError("order 1 or 2 (order 4 canonises with OrthogonaliseInversionSymmetric)")
+
+
+def evolve(ops, order, cell, steps, D=0, minS=0.0):
+    """iTEBD::evolve (TEBD_routines.cpp:120-185) for orders 1 and 2 with no
+    measurement step inside the run.  ``cell`` = [A0, A1, lambda0, lambda1];
+    returns the new cell and the truncation of the last gate.
+
+    Second order: odd(tau/2), then per step even(tau) and odd(tau), closing
+    with even(tau), odd(tau/2) and the cell swap.  The reference's
+    no-measurement branch applies OrderedOperatorPtrs[1] (even) where the
+    odd full-step operator [2] belongs (TEBD_routines.cpp:163); its
+    measuring branch applies odd(tau/2) twice around the canonisation, which
+    is the same odd(tau) up to truncation.  This restatement uses [2]."""
+    A0, A1, l0, l1 = cell
+    tr = 0.0
+
+    def gate(G):
+        nonlocal A0, A1, l0, l1, tr
+        nA0, nA1, nl0, tr = apply_and_decompose(G, A0, A1, l0, D, minS)
+        A0, A1, l0, l1 = nA0, nA1, nl0, l0
+
+    if order == 1:
+        for _ in range(steps):
+            gate(ops[0])
+            gate(ops[1])
+    elif order == 2:
+        if steps > 0:
+            gate(ops[0])
+            for _ in range(steps - 1):
+                gate(ops[1])
+                gate(ops[2])
+            gate(ops[1])
+            gate(ops[0])
+            A0, A1, l0, l1 = A1, A0, l1, l0         # m_unit.swap(0,1)
+    else:
+        raise ValueError("order 1 or 2")
+    return [A0, A1, l0, l1], tr

@@ -96,6 +96,28 @@ def test_heff_launch_configurations(ctx, algo, tile_mode, wg):
     assert rel_err(H.apply(v, cam.CAM_FP32), ref) <= 2e-5
 
 
+def test_heff_instances_share_workspace(ctx):
+    """Two H_eff on one context share the grow-only intermediate buffers: a
+    smaller one built first still computes correctly after a larger one has
+    grown (re-allocated) them, and so does the larger one afterwards."""
+    print("case heff_shared_workspace", flush=True)
+    import chainamps_amd as cam
+    small = CASES["heisenberg"]()
+    big = CASES["random_u1_w6"]()
+    rng = np.random.default_rng(13)
+    Hs = cam.Heff(*(to_lib(ctx, t) for t in (small[1], small[2], small[3])))
+    comp_s = dmrg.TwoVertexComponents(small[1], small[2], small[3])
+    vs = rng.standard_normal(comp_s.n) + 1j * rng.standard_normal(comp_s.n)
+    assert rel_err(Hs.apply(vs), comp_s.matvec(vs)) <= 1e-12
+    Hb = cam.Heff(*(to_lib(ctx, t) for t in (big[1], big[2], big[3])))
+    comp_b = dmrg.TwoVertexComponents(big[1], big[2], big[3])
+    vb = rng.standard_normal(comp_b.n) + 1j * rng.standard_normal(comp_b.n)
+    assert rel_err(Hb.apply(vb), comp_b.matvec(vb)) <= 1e-12
+    assert rel_err(Hs.apply(vs), comp_s.matvec(vs)) <= 1e-12
+    assert rel_err(Hs.apply(vs, cam.CAM_FP32), comp_s.matvec(vs)) <= 2e-5
+    assert rel_err(Hb.apply(vb), comp_b.matvec(vb)) <= 1e-12
+
+
 def test_heff_hermitian_and_linear(ctx):
     print("case heff_hermitian", flush=True)
     import chainamps_amd as cam
