@@ -430,7 +430,8 @@ void GemmPlan::build_tiles_mode(int mode) {
     int shape = 0;
     if (mode != 0) {
       double best = 1e300;
-      for (int t = (mode == 2 ? 1 : 0); t < kNumTileShapes; ++t) {
+      const int t_end = mode == 3 ? kNumTileShapes : kNumSmallShapes;
+      for (int t = (mode == 2 ? 1 : 0); t < t_end; ++t) {
         const double tm = kTileShapes[t].tm, tn = kTileShapes[t].tn;
         const double pm = std::ceil(p.M / tm) * tm, pn = std::ceil(p.N / tn) * tn;
         // padded area, with a small per-tile overhead so bigger tiles win ties
@@ -576,7 +577,7 @@ bool autotune(GemmPlan& plan, int precision, hipStream_t s, const GemmBases* bas
     }
   };
   try_axis(&LaunchCfg::algo, {kAlgo3M, kAlgo4M});
-  try_axis(&LaunchCfg::tile_mode, {1, 0, 2});
+  try_axis(&LaunchCfg::tile_mode, {1, 0, 2, 3});
   try_axis(&LaunchCfg::wg_per_cu, {4, 2, 8});
   apply_choice(plan, precision, cur, s);
   // leave the output of the chosen configuration (every candidate computes
@@ -603,10 +604,14 @@ void launch_grouped_gemm(GemmPlan& plan, int precision, hipStream_t s, const Gem
     launch_shape<double, 64, 64>(plan, c, 0, s, grid_limit, b);
     launch_shape<double, 32, 64>(plan, c, 1, s, grid_limit, b);
     launch_shape<double, 64, 32>(plan, c, 2, s, grid_limit, b);
+    launch_shape<double, 128, 64>(plan, c, 3, s, grid_limit, b);
+    launch_shape<double, 64, 128>(plan, c, 4, s, grid_limit, b);
   } else {
     launch_shape<float, 64, 64>(plan, c, 0, s, grid_limit, b);
     launch_shape<float, 32, 64>(plan, c, 1, s, grid_limit, b);
     launch_shape<float, 64, 32>(plan, c, 2, s, grid_limit, b);
+    launch_shape<float, 128, 64>(plan, c, 3, s, grid_limit, b);
+    launch_shape<float, 64, 128>(plan, c, 4, s, grid_limit, b);
   }
 }
 

@@ -205,7 +205,29 @@ inline int grid_for(int64_t n) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g, 8192)));
 }
 
+__global__ void random_fill_kernel(double2* x, int64_t n, uint64_t seed) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double a = (z >> 11) * (1.0 / 9007199254740992.0);
+    z = z * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z ^= z >> 29;
+    const double b = (z >> 11) * (1.0 / 9007199254740992.0);
+    x[i] = make_double2(2.0 * a - 1.0, 2.0 * b - 1.0);
+  }
+}
+
 }  // namespace
+
+void launch_random_fill(cplx* x, int64_t n, uint64_t seed, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(random_fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, reinterpret_cast<double2*>(x), n,
+                     seed);
+  HIP_CHECK(hipGetLastError());
+}
 
 void launch_permute(const std::vector<PermJob>& jobs, DevBuf<PermJob>& djobs, DevBuf<int64_t>& dchunks,
                     hipStream_t s) {

@@ -53,4 +53,8 @@ void launch_combine(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, i
                     hipStream_t s);
 int multi_dot_blocks(int64_t n);
 
+// x[i] = (a, b), a and b uniform in [-1, 1) from a counter-based splitmix64
+// stream (deterministic for a given seed)
+void launch_random_fill(cplx* x, int64_t n, uint64_t seed, hipStream_t s);
+
 }  // namespace cam

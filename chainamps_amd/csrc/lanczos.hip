@@ -51,21 +51,6 @@ Ctx::~Ctx() {
 
 namespace {
 
-__global__ void random_fill_kernel(double2* x, int64_t n, uint64_t seed) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    uint64_t z = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    const double a = (z >> 11) * (1.0 / 9007199254740992.0);
-    z = z * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
-    z ^= z >> 29;
-    const double b = (z >> 11) * (1.0 / 9007199254740992.0);
-    x[i] = make_double2(2.0 * a - 1.0, 2.0 * b - 1.0);
-  }
-}
-
 // Lanczos step tail on the device: beta = |w| from the norm slot; the step
 // breaks down when beta <= 16 eps max(1, |alpha|), alpha = Re of the summed
 // CGS2 coefficients c1[j] + c2[j].  Writes v_{j+1} = w / beta (zero after a
@@ -355,11 +340,7 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
   if (have_start) {
     have_start = L.norm(xout) > 0.0;
   }
-  if (!have_start) {
-    hipLaunchKernelGGL(random_fill_kernel, dim3(1024), dim3(256), 0, s,
-                       reinterpret_cast<double2*>(xout), n, 0x1234ull);
-    HIP_CHECK(hipGetLastError());
-  }
+  if (!have_start) launch_random_fill(xout, n, 0x1234ull, s);
   {
     const double nx = L.norm(xout);
     launch_scale(cplx{1.0 / nx, 0.0}, xout, n, s);

@@ -25,6 +25,13 @@ void gauge_fix(cplx* x, int64_t n, hipStream_t s);
 // contract with a diagonal on index k (values per position of that index)
 void scale_along_index(Ctx& ctx, BSTensor& T, int k, const std::vector<double>& lam);
 
+// Leading k singular triplets of a dense m x n block (column-major, device)
+// by randomized subspace iteration; true when the Ritz residuals of the
+// first ncheck triplets are <= tol within max_rounds rounds.  U is m x k,
+// Vh is k x n, S descending.
+bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int64_t ncheck, double tol,
+                   int max_rounds, std::vector<double>& S, DevBuf<cplx>& U, DevBuf<cplx>& Vh, int* rounds);
+
 // MPX_matrix::SVD with the reference truncation rule.
 void svd_truncate(Ctx& ctx, const BSTensor& theta, int64_t D, double max_trunc, TensorPtr& U,
                   TensorPtr& Vd, std::vector<double>& values, double& discarded);

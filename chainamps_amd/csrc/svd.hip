@@ -25,6 +25,7 @@
 #include <cmath>
 #include <limits>
 #include <map>
+#include <cstdlib>
 #include <numeric>
 
 #include "solvers.hpp"
@@ -32,6 +33,13 @@
 namespace cam {
 
 namespace {
+
+// Blocks with min(rows, cols) >= this and room for at most half their values
+// use the leading-values decomposition (CAM_SVD_EXACT=1 disables it).
+constexpr int64_t kTruncatedMinDim = 1536;
+// Ritz residual bound of accepted values, relative to the block's
+// Frobenius norm (an absolute bound on each value's error).
+constexpr double kRitzTol = 2e-15;
 
 struct FlagJob {
   const double2* in;
@@ -117,6 +125,8 @@ struct Group {
   int64_t k = 0;                                 // number of singular values
   DevBuf<cplx> A, U, Vh;
   std::vector<double> S;
+  bool truncated = false;                        // only the leading k values computed
+  double tail = 0.0;                             // weight - sum S^2 when truncated
 };
 
 }  // namespace
@@ -220,6 +230,33 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
   DevBuf<int> dinfo(1);
   DevBuf<double> dE, dS;
   DevBuf<cplx> partial(64 * 1024), dcoef(64);
+  const char* ev = std::getenv("CAM_SVD_EXACT");
+  const bool exact_only = ev && *ev && *ev != '0';
+  // the full (economy) decomposition of one group, as the reference's
+  // per-block zgesdd
+  auto full_svd = [&](Group& g) {
+    g.k = std::min(g.mg, g.ng);
+    g.truncated = false;
+    g.tail = 0.0;
+    g.U.alloc(g.mg * g.k);
+    g.Vh.alloc(g.k * g.ng);
+    dS.ensure(g.k);
+    dE.ensure(std::max<int64_t>(g.k, 1));
+    const rocblas_status st = rocsolver_zgesvd(
+        h, rocblas_svect_singular, rocblas_svect_singular, static_cast<rocblas_int>(g.mg),
+        static_cast<rocblas_int>(g.ng), reinterpret_cast<rocblas_double_complex*>(g.A.get()),
+        static_cast<rocblas_int>(g.mg), dS.get(), reinterpret_cast<rocblas_double_complex*>(g.U.get()),
+        static_cast<rocblas_int>(g.mg), reinterpret_cast<rocblas_double_complex*>(g.Vh.get()),
+        static_cast<rocblas_int>(g.k), dE.get(), rocblas_outofplace, dinfo.get());
+    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgesvd failed");
+    g.S.resize(g.k);
+    int info = 0;
+    HIP_CHECK(hipMemcpyAsync(g.S.data(), dS.get(), g.k * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    CAM_REQUIRE(info == 0, "blockwise SVD did not converge");
+    g.A.reset();
+  };
   for (size_t gi = 0; gi < glist.size(); ++gi) {
     Group& g = *glist[gi];
     if (g.mg == 0 || g.ng == 0) continue;
@@ -273,42 +310,54 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
     HIP_CHECK(hipStreamSynchronize(s));
     g.weight = wsum.re;
     if (g.weight <= eps * eps) continue;                  // empty block: skipped
-    g.k = std::min(g.mg, g.ng);
-    g.U.alloc(g.mg * g.k);
-    g.Vh.alloc(g.k * g.ng);
-    dS.ensure(g.k);
-    dE.ensure(std::max<int64_t>(g.k, 1));
-    const rocblas_status st = rocsolver_zgesvd(
-        h, rocblas_svect_singular, rocblas_svect_singular, static_cast<rocblas_int>(g.mg),
-        static_cast<rocblas_int>(g.ng), reinterpret_cast<rocblas_double_complex*>(g.A.get()),
-        static_cast<rocblas_int>(g.mg), dS.get(), reinterpret_cast<rocblas_double_complex*>(g.U.get()),
-        static_cast<rocblas_int>(g.mg), reinterpret_cast<rocblas_double_complex*>(g.Vh.get()),
-        static_cast<rocblas_int>(g.k), dE.get(), rocblas_outofplace, dinfo.get());
-    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgesvd failed");
-    g.S.resize(g.k);
-    int info = 0;
-    HIP_CHECK(hipMemcpyAsync(g.S.data(), dS.get(), g.k * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    CAM_REQUIRE(info == 0, "blockwise SVD did not converge");
-    g.A.reset();
+    // a large block of which at most D values can be kept: leading values
+    // only (randomized subspace iteration, accepted on its Ritz residuals)
+    const int64_t kfull = std::min(g.mg, g.ng);
+    const int64_t kk = D > 0 ? std::min(kfull, D + std::max<int64_t>(D / 2, 32)) : kfull;
+    if (!exact_only && D > 0 && kfull >= kTruncatedMinDim && 2 * kk <= kfull) {
+      std::vector<double> S;
+      int rounds = 0;
+      if (truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, std::min(D, kk), kRitzTol * std::sqrt(g.weight), 24, S,
+                        g.U, g.Vh, &rounds)) {
+        g.S = std::move(S);
+        g.k = kk;
+        g.truncated = true;
+        double kept = 0.0;
+        for (double v : g.S) kept += v * v;
+        g.tail = std::max(0.0, g.weight - kept);
+        continue;   // g.A is kept in case the full decomposition is needed
+      }
+    }
+    full_svd(g);
   }
 
   // ---- global sort and truncation rule --------------------------------------
+  // A truncated group contributes its leading values; the weight of its
+  // uncomputed tail enters the discarded weight, which is exactly the sum
+  // the full rule would accumulate over those values as long as every one
+  // of them lies below the kept set.  That holds when each truncated
+  // group's smallest computed value is below the last kept value by more
+  // than the 1e-3 degeneracy gap; otherwise the group is decomposed in full
+  // and the rule is applied again.
   struct SV { double v; int g; int64_t i; };
   std::vector<SV> all;
-  for (size_t gi = 0; gi < glist.size(); ++gi)
-    for (int64_t i = 0; i < glist[gi]->k; ++i) all.push_back(SV{glist[gi]->S[i], static_cast<int>(gi), i});
-  CAM_REQUIRE(!all.empty(), "Error: No singular values. Empty blocks?");
-  std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) {
-    if (a.v != b.v) return a.v > b.v;
-    if (a.g != b.g) return a.g < b.g;
-    return a.i < b.i;
-  });
-  const size_t nall = all.size();
-  size_t length = (D > 0 && static_cast<size_t>(D) < nall) ? static_cast<size_t>(D) : nall;
-  {
-    double trunc_error = 0.0;
+  size_t length = 0;
+  for (int pass = 0;; ++pass) {
+    all.clear();
+    double tail = 0.0;
+    for (size_t gi = 0; gi < glist.size(); ++gi) {
+      for (int64_t i = 0; i < glist[gi]->k; ++i) all.push_back(SV{glist[gi]->S[i], static_cast<int>(gi), i});
+      if (glist[gi]->truncated) tail += glist[gi]->tail;
+    }
+    CAM_REQUIRE(!all.empty(), "Error: No singular values. Empty blocks?");
+    std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) {
+      if (a.v != b.v) return a.v > b.v;
+      if (a.g != b.g) return a.g < b.g;
+      return a.i < b.i;
+    });
+    const size_t nall = all.size();
+    length = (D > 0 && static_cast<size_t>(D) < nall) ? static_cast<size_t>(D) : nall;
+    double trunc_error = tail;
     size_t tl = nall;
     while (trunc_error < max_trunc && tl > 0) {
       trunc_error += all[tl - 1].v * all[tl - 1].v;
@@ -317,11 +366,20 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
     ++tl;
     if (tl < length) length = tl;
     while (length < nall && (all[length - 1].v - all[length].v) / all[length - 1].v < 1.0e-3) ++length;
+    discarded = tail;
+    for (size_t i = length; i < nall; ++i) discarded += all[i].v * all[i].v;
+    bool redo = false;
+    const double last = all[length - 1].v;
+    for (Group* g : glist)
+      if (g->truncated && !(g->S.back() < last * (1.0 - 2.0e-3))) {
+        full_svd(*g);   // each pass turns at least one group exact: at most #groups passes
+        redo = true;
+      }
+    if (!redo) break;
   }
+  for (Group* g : glist) g->A.reset();
   values.clear();
   for (size_t i = 0; i < length; ++i) values.push_back(all[i].v);
-  discarded = 0.0;
-  for (size_t i = length; i < nall; ++i) discarded += all[i].v * all[i].v;
 
   // ---- new bond index and the U / V^dagger tensors ------------------------
   QIndex nb;

@@ -128,7 +128,8 @@ int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, dou
  * histogram; CAM_AUTOTUNE=0 disables).  set fixes it for every stage:
  * algo 0 = four real MFMAs per complex product, 1 = three (Gauss);
  * tile_mode 0 = 64x64 tiles, 1 = per-block least padding, 2 = 32x64/64x32
- * only; wg_per_cu = resident workgroups per CU of the persistent grid.
+ * only, 3 = least padding including 128x64 / 64x128 tiles; wg_per_cu =
+ * workgroups per CU in the persistent grid.
  * get returns {algo, tile_mode, wg_per_cu} per stage (12 ints). */
 int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu);
 int cam_heff_get_launch(const cam_heff* h, int precision, int* cfg);

@@ -18,8 +18,8 @@ Restates the reference's general tensor class and its index choreography:
     sparse_interface.cpp:1248-1406).
 
 The SpGEMM itself is scipy's (a Gustavson/SMMP product, the same algorithm
-family as sparse_interface.cpp:1892-2170); the C restatement in
-``oracle/spgemm.c`` is used for the timed CPU baseline.
+family as sparse_interface.cpp:1892-2170, single-threaded); the timed CPU
+baseline runs the reference-order matvec through it on one core.
 
 Parity pins (tests/test_oracle_mpx.py): every reshape/contract result is
 checked against numpy.einsum on the dense tensors built from the mixed-radix

@@ -76,7 +76,8 @@ def test_heff_matvec_parity(ctx, case):
         assert rel_err(out32, ref) <= 2e-5, (case, trial)
 
 
-@pytest.mark.parametrize("algo,tile_mode,wg", [(0, 1, 4), (1, 1, 4), (1, 0, 2), (0, 2, 8), (1, 2, 1)])
+@pytest.mark.parametrize("algo,tile_mode,wg", [(0, 1, 4), (1, 1, 4), (1, 0, 2), (0, 2, 8), (1, 2, 1),
+                                               (1, 3, 2), (0, 3, 4)])
 def test_heff_launch_configurations(ctx, algo, tile_mode, wg):
     """Every launch configuration the autotuner can pick (four or three real
     MFMAs per complex product, each tile policy, grid sizes) gives the same

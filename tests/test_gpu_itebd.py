@@ -36,10 +36,16 @@ def test_itebd_evolve_vs_oracle(ctx, order):
             assert len(got) == len(cell[2 + k]), (chunk, k)
             assert np.allclose(got, cell[2 + k], rtol=1e-10, atol=1e-12), (chunk, k)
         assert abs(rec["truncation"] - tr) <= 1e-10
+        # singular vectors carry a free phase per bond state (rocSOLVER and
+        # LAPACK choose differently), so compare the two-site reduced density
+        # matrix sum_{a,b} T[s0,a,s1,b] conj(T[s0',a,s1',b]) of T = A0 A1,
+        # which is invariant under any unitary on either bond
         g0, g1 = from_lib(dev.tensor(0), rules), from_lib(dev.tensor(1), rules)
-        two = O.contract(g0, False, g1, False, [(2, 1)]).to_dense_tensor()
-        ref = O.contract(cell[0], False, cell[1], False, [(2, 1)]).to_dense_tensor()
-        assert rel_err(two, ref) <= 1e-9, chunk
+        T = O.contract(g0, False, g1, False, [(2, 1)]).to_dense_tensor()
+        Tr = O.contract(cell[0], False, cell[1], False, [(2, 1)]).to_dense_tensor()
+        rho = np.einsum("iajb,kalb->ijkl", T, T.conj())
+        rho_ref = np.einsum("iajb,kalb->ijkl", Tr, Tr.conj())
+        assert rel_err(rho, rho_ref) <= 1e-9, chunk
 
 
 def test_itebd_unitary_without_truncation(ctx):
