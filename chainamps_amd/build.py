@@ -71,7 +71,31 @@ def build(jobs: int = 0, debug: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, out)
+    build_cpp_examples(out)
     return out
+
+
+def build_cpp_examples(lib: Path) -> list[Path]:
+    """The C++ programs under tests/cpp use the header-only layer
+    include/chainamps_amd.hpp (the reference-shaped API) over the library;
+    they are host C++ (g++, C++11) linked against it."""
+    root = HERE.parent
+    srcs = sorted((root / "tests" / "cpp").glob("*.cpp"))
+    gxx = shutil.which("g++")
+    if not srcs or not gxx:
+        return []
+    outs = []
+    for src in srcs:
+        exe = LIBDIR / src.stem
+        tmp = LIBDIR / (src.stem + ".tmp")
+        cmd = [gxx, "-std=c++11", "-O2", "-Wall", f"-I{root / 'include'}", str(src), "-o", str(tmp),
+               f"-L{LIBDIR}", "-lchainamps_amd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, exe)
+        outs.append(exe)
+    return outs
 
 
 def main(argv=None):

@@ -1,0 +1,448 @@
+/*
+ * chainamps_amd.hpp -- C++ host layer over the C ABI, mirroring the
+ * reference's own interface for the hot path (namespace ajaj in
+ * MPX_matrix.hpp, DMRG_routines.hpp, TEBD_routines.hpp) so that a caller of
+ * ChainAMPS keeps its call sites: MPX_matrix / MPXIndex / MPXDecomposition,
+ * contract(), reorder(), MPX_matrix::SVD(), TwoVertexComponents::HED(),
+ * TwoVertexMPOMPSMultiply(), MakeOdd/EvenBondHamiltonian(),
+ * MakeBondEvolutionOperator(), iTEBD::evolve() and the iDMRG growth step.
+ *
+ * Tensors live on the device (one MI355X); host data crosses only where the
+ * reference's signatures take host data (CSC construction, eigenvectors,
+ * matvec buffers).  Errors follow the reference: the message is printed to
+ * std::cout and the process exits with status 1; define CAM_AJAJ_THROW
+ * before including this header to get a std::runtime_error instead.
+ *
+ * Header only; link with -lchainamps_amd.  C++11.
+ */
+#ifndef CHAINAMPS_AMD_HPP
+#define CHAINAMPS_AMD_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "chainamps_amd.h"
+
+namespace cam_ajaj {
+
+typedef long long MPXInt;
+typedef std::pair<MPXInt, MPXInt> MPXPair;   // (index of A, index of B) contracted together
+typedef std::vector<short> State;            // one charge per rule (states.hpp:32-61)
+typedef std::vector<State> StateArray;
+
+inline void check(int rc) {
+  if (rc == 0) return;
+  const std::string msg = cam_last_error();
+#ifdef CAM_AJAJ_THROW
+  throw std::runtime_error(msg);
+#else
+  std::cout << msg << std::endl;
+  std::exit(1);
+#endif
+}
+
+/* One device context per process thread (the reference is single-threaded on
+ * the host side). */
+class Context {
+ public:
+  static cam_ctx* get(int device = 0) {
+    static Context c(device);
+    return c.ctx_;
+  }
+  ~Context() {
+    if (ctx_) cam_ctx_destroy(ctx_);
+  }
+
+ private:
+  explicit Context(int device) { check(cam_ctx_create(device, &ctx_)); }
+  cam_ctx* ctx_ = nullptr;
+};
+
+/* MPXIndex (MPXIndex.hpp:17-65): direction, physical flag, charges. */
+class MPXIndex {
+ public:
+  MPXIndex(bool inward, const StateArray& states, bool physical = false)
+      : inward_(inward), physical_(physical), states_(states) {}
+  bool Ingoing() const { return inward_; }
+  bool Outgoing() const { return !inward_; }
+  bool Physical() const { return physical_; }
+  MPXInt size() const { return static_cast<MPXInt>(states_.size()); }
+  const State& at(MPXInt i) const { return states_.at(static_cast<size_t>(i)); }
+  const StateArray& states() const { return states_; }
+  MPXIndex flip() const { return MPXIndex(!inward_, states_, physical_); }
+
+ private:
+  bool inward_, physical_;
+  StateArray states_;
+};
+
+class MPX_matrix;
+struct MPXDecomposition;
+MPX_matrix contract(const MPX_matrix& A, bool conjA, const MPX_matrix& B, bool conjB,
+                    const std::vector<MPXPair>& contractidxs);
+
+/* MPX_matrix (MPX_matrix.hpp:61-147): a QN-blocked tensor, resident on the
+ * device.  `rules` are the charge rules (states.hpp:32-34: N > 1 is Z_N,
+ * N == 1 trivial, N <= 0 U(1)). */
+class MPX_matrix {
+ public:
+  MPX_matrix() = default;
+  /* From the reference's storage: indices left to right, numrowindices of
+   * them form the CSC rows (mixed radix, first index fastest); x holds
+   * complex values, p has cols+1 entries. */
+  MPX_matrix(const std::vector<short>& rules, const std::vector<MPXIndex>& indices, MPXInt numrowindices,
+             MPXInt rows, MPXInt cols, const std::vector<long long>& p, const std::vector<long long>& i,
+             const std::vector<std::complex<double>>& x) {
+    std::vector<cam_index_desc> d;
+    std::vector<std::vector<int16_t>> q(indices.size());
+    for (size_t k = 0; k < indices.size(); ++k) {
+      for (const State& s : indices[k].states()) q[k].insert(q[k].end(), s.begin(), s.end());
+      cam_index_desc e;
+      e.inward = indices[k].Ingoing();
+      e.physical = indices[k].Physical();
+      e.size = indices[k].size();
+      e.charges = q[k].data();
+      d.push_back(e);
+    }
+    std::vector<int64_t> pp(p.begin(), p.end()), ii(i.begin(), i.end());
+    if (ii.empty()) ii.push_back(0);
+    std::vector<std::complex<double>> xx(x);
+    if (xx.empty()) xx.push_back(0.0);
+    std::vector<int16_t> r(rules.begin(), rules.end());
+    cam_tensor* t = nullptr;
+    check(cam_tensor_from_csc(Context::get(), static_cast<int>(r.size()), r.data(), static_cast<int>(d.size()),
+                              d.data(), static_cast<int>(numrowindices), rows, cols, pp.data(), ii.data(),
+                              reinterpret_cast<const double*>(xx.data()), &t));
+    reset(t);
+  }
+  /* Dense column-major (rows x cols) values, exact zeros not stored. */
+  static MPX_matrix FromDense(const std::vector<short>& rules, const std::vector<MPXIndex>& indices,
+                              MPXInt numrowindices, const std::vector<std::complex<double>>& dense) {
+    MPXInt rows = 1, cols = 1;
+    for (MPXInt k = 0; k < numrowindices; ++k) rows *= indices[static_cast<size_t>(k)].size();
+    for (size_t k = static_cast<size_t>(numrowindices); k < indices.size(); ++k) cols *= indices[k].size();
+    std::vector<long long> p(static_cast<size_t>(cols) + 1, 0), i;
+    std::vector<std::complex<double>> x;
+    for (MPXInt c = 0; c < cols; ++c) {
+      for (MPXInt r = 0; r < rows; ++r) {
+        const std::complex<double> v = dense.at(static_cast<size_t>(r + c * rows));
+        if (v != 0.0) {
+          i.push_back(r);
+          x.push_back(v);
+        }
+      }
+      p[static_cast<size_t>(c) + 1] = static_cast<long long>(i.size());
+    }
+    return MPX_matrix(rules, indices, numrowindices, rows, cols, p, i, x);
+  }
+
+  bool isEmpty() const { return !t_; }
+  cam_tensor* handle() const { return t_.get(); }
+  MPXInt NumRowIndices() const {
+    int n = 0, r = 0, nr = 0;
+    check(cam_tensor_shape(t_.get(), &n, &r, &nr));
+    return r;
+  }
+  MPXIndex Index(MPXInt k) const {
+    int n = 0, r = 0, nr = 0;
+    check(cam_tensor_shape(t_.get(), &n, &r, &nr));
+    int32_t inw = 0, ph = 0;
+    int64_t size = 0;
+    check(cam_tensor_index(t_.get(), static_cast<int>(k), &inw, &ph, &size, nullptr));
+    std::vector<int16_t> q(static_cast<size_t>(size * nr));
+    check(cam_tensor_index(t_.get(), static_cast<int>(k), nullptr, nullptr, nullptr, q.data()));
+    StateArray s(static_cast<size_t>(size), State(static_cast<size_t>(nr)));
+    for (int64_t e = 0; e < size; ++e)
+      for (int c = 0; c < nr; ++c) s[static_cast<size_t>(e)][static_cast<size_t>(c)] = q[static_cast<size_t>(e * nr + c)];
+    return MPXIndex(inw != 0, s, ph != 0);
+  }
+  std::vector<MPXInt> dimsvector() const {
+    int n = 0, r = 0, nr = 0;
+    check(cam_tensor_shape(t_.get(), &n, &r, &nr));
+    std::vector<MPXInt> d;
+    for (int k = 0; k < n; ++k) {
+      int64_t size = 0;
+      check(cam_tensor_index(t_.get(), k, nullptr, nullptr, &size, nullptr));
+      d.push_back(size);
+    }
+    return d;
+  }
+  /* Host CSC copy (exact zeros dropped). */
+  void to_csc(std::vector<long long>& p, std::vector<long long>& i, std::vector<std::complex<double>>& x) const {
+    int64_t nnz = 0;
+    MPXInt cols = 1;
+    const std::vector<MPXInt> d = dimsvector();
+    for (size_t k = static_cast<size_t>(NumRowIndices()); k < d.size(); ++k) cols *= d[k];
+    std::vector<int64_t> pp(static_cast<size_t>(cols) + 1);
+    check(cam_tensor_to_csc(t_.get(), 0, nullptr, nullptr, nullptr, &nnz));   // size only
+    std::vector<int64_t> ii(static_cast<size_t>(std::max<int64_t>(nnz, 1)));
+    x.assign(static_cast<size_t>(std::max<int64_t>(nnz, 1)), 0.0);
+    check(cam_tensor_to_csc(t_.get(), nnz, pp.data(), ii.data(), reinterpret_cast<double*>(x.data()), &nnz));
+    p.assign(pp.begin(), pp.end());
+    i.assign(ii.begin(), ii.begin() + nnz);
+    x.resize(static_cast<size_t>(nnz));
+  }
+
+  MPX_matrix& ShiftNumRowIndices(MPXInt numrows) {
+    check(cam_tensor_shift_num_row_indices(t_.get(), static_cast<int>(numrows)));
+    return *this;
+  }
+  MPX_matrix& RemoveDummyIndices(const std::vector<MPXInt>& which) {
+    std::vector<int64_t> w(which.begin(), which.end());
+    check(cam_tensor_remove_dummy_indices(t_.get(), static_cast<int>(w.size()), w.data()));
+    return *this;
+  }
+  MPX_matrix& MoveDummyIndices(const std::vector<MPXInt>& order) {
+    std::vector<int64_t> o(order.begin(), order.end());
+    check(cam_tensor_move_dummy_indices(t_.get(), static_cast<int>(o.size()), o.data()));
+    return *this;
+  }
+  MPX_matrix& Rescale(std::complex<double> factor) {
+    check(cam_tensor_rescale(t_.get(), factor.real(), factor.imag()));
+    return *this;
+  }
+  /* MPX_matrix::SVD (MPX_matrix.hpp:116). */
+  MPXDecomposition SVD(size_t bond_dimension = 0, double min_s_val = 0.0) const;
+
+  explicit MPX_matrix(cam_tensor* t) { reset(t); }
+
+ private:
+  void reset(cam_tensor* t) {
+    t_ = std::shared_ptr<cam_tensor>(t, [](cam_tensor* p) {
+      if (p) cam_tensor_destroy(p);
+    });
+  }
+  std::shared_ptr<cam_tensor> t_;
+};
+
+/* MPXDecomposition (MPX_matrix.hpp:152-184): ColumnMatrix = U,
+ * RowMatrix = V^dagger. */
+struct MPXDecomposition {
+  double Truncation = 0.0;
+  MPX_matrix ColumnMatrix;
+  std::vector<double> Values;
+  MPX_matrix RowMatrix;
+  const std::vector<double>& SquareRescale(double sqsum) {
+    double w = 0.0;
+    for (double v : Values) w += v * v;
+    const double f = std::sqrt(sqsum / w);
+    for (double& v : Values) v *= f;
+    Truncation /= sqsum;
+    return Values;
+  }
+};
+
+inline MPXDecomposition MPX_matrix::SVD(size_t bond_dimension, double min_s_val) const {
+  cam_tensor *U = nullptr, *Vd = nullptr;
+  int64_t n = 0;
+  double disc = 0.0;
+  // at most min(rows, cols) values
+  const std::vector<MPXInt> dims = dimsvector();
+  const MPXInt nr = NumRowIndices();
+  MPXInt rows = 1, cols = 1;
+  for (size_t k = 0; k < dims.size(); ++k) (static_cast<MPXInt>(k) < nr ? rows : cols) *= dims[k];
+  std::vector<double> vals(static_cast<size_t>(std::max<MPXInt>(std::min(rows, cols), 1)));
+  check(cam_svd_truncate(Context::get(), t_.get(), static_cast<int64_t>(bond_dimension), min_s_val, &U, &Vd,
+                         vals.data(), static_cast<int64_t>(vals.size()), &n, &disc));
+  MPXDecomposition d;
+  d.Truncation = disc;
+  d.ColumnMatrix = MPX_matrix(U);
+  d.RowMatrix = MPX_matrix(Vd);
+  d.Values.assign(vals.begin(), vals.begin() + n);
+  return d;
+}
+
+/* ajaj::contract (MPX_matrix.hpp:25, MPX_matrix.cpp:744-791). */
+inline MPX_matrix contract(const MPX_matrix& A, bool conjA, const MPX_matrix& B, bool conjB,
+                           const std::vector<MPXPair>& contractidxs) {
+  std::vector<int64_t> flat;
+  for (const MPXPair& p : contractidxs) {
+    flat.push_back(p.first);
+    flat.push_back(p.second);
+  }
+  cam_tensor* C = nullptr;
+  check(cam_contract(Context::get(), A.handle(), conjA, B.handle(), conjB, static_cast<int>(contractidxs.size()),
+                     flat.data(), &C));
+  return MPX_matrix(C);
+}
+inline MPX_matrix contract(const MPX_matrix& A, bool conjA, const MPX_matrix& B, bool conjB,
+                           const MPXPair& contractidx) {
+  return contract(A, conjA, B, conjB, std::vector<MPXPair>(1, contractidx));
+}
+
+/* ajaj::reorder (MPX_matrix.cpp:630-638). */
+inline MPX_matrix reorder(const MPX_matrix& A, bool conjA, const std::vector<MPXInt>& neworder,
+                          MPXInt numrowindices) {
+  std::vector<int64_t> o(neworder.begin(), neworder.end());
+  cam_tensor* C = nullptr;
+  check(cam_reorder(Context::get(), A.handle(), conjA, static_cast<int>(o.size()), o.data(),
+                    static_cast<int>(numrowindices), &C));
+  return MPX_matrix(C);
+}
+
+/* SparseHED as returned by TwoVertexComponents::HED: eigenvalues and the
+ * eigenvector in the compressed (allowed_indices) order. */
+struct SparseHED {
+  std::vector<double> Values;
+  std::vector<std::complex<double>> EigenVector;
+  double info[4] = {0, 0, 0, 0};   // matvecs fp64, matvecs fp32, restarts, residual
+};
+
+/* TwoVertexComponents (DMRG_routines.hpp:189-214). */
+class TwoVertexComponents {
+ public:
+  TwoVertexComponents(const MPX_matrix& LeftBlock, const MPX_matrix& H, const MPX_matrix& RightBlock) {
+    cam_heff* h = nullptr;
+    int64_t n = 0;
+    check(cam_heff_create(Context::get(), LeftBlock.handle(), H.handle(), RightBlock.handle(), &h, &n));
+    h_ = std::shared_ptr<cam_heff>(h, [](cam_heff* p) {
+      if (p) cam_heff_destroy(p);
+    });
+    n_ = n;
+  }
+  MPXInt size() const { return n_; }
+  std::vector<long long> allowed_indices() const {
+    std::vector<int64_t> a(static_cast<size_t>(n_));
+    check(cam_heff_allowed_map(h_.get(), a.data()));
+    return std::vector<long long>(a.begin(), a.end());
+  }
+  /* TwoVertexComponents::HED (DMRG_routines.cpp:835-896): numevals = 1,
+   * which = "SR"; initial: guess in compressed order or nullptr;
+   * converge = false is the growth/sweep setting (tol 0.1).  mixed selects
+   * the FP32-split bulk with FP64 refinement (an addition; false is the
+   * reference path). */
+  SparseHED HED(MPXInt numevals, const char which[3], const std::vector<std::complex<double>>* initial,
+                bool converge, bool mixed = false) const {
+    if (numevals != 1 || std::string(which).substr(0, 2) != "SR") {
+      const std::string m = "TwoVertexComponents::HED: only the lowest (numevals=1, which=SR) eigenpair";
+#ifdef CAM_AJAJ_THROW
+      throw std::runtime_error(m);
+#else
+      std::cout << m << std::endl;
+      std::exit(1);
+#endif
+    }
+    SparseHED ans;
+    ans.EigenVector.assign(static_cast<size_t>(n_), 0.0);
+    double e = 0.0;
+    check(cam_heff_lowest(h_.get(), initial ? reinterpret_cast<const double*>(initial->data()) : nullptr, converge,
+                          0.0, 20, mixed, &e, reinterpret_cast<double*>(ans.EigenVector.data()), ans.info));
+    ans.Values.push_back(e);
+    return ans;
+  }
+  cam_heff* handle() const { return h_.get(); }
+
+ private:
+  std::shared_ptr<cam_heff> h_;
+  MPXInt n_ = 0;
+};
+
+/* TwoVertexMPOMPSMultiply (DMRG_routines.cpp:898-974): out = H_eff in, host
+ * buffers of length size() in the compressed order. */
+inline void TwoVertexMPOMPSMultiply(const TwoVertexComponents* comp, std::complex<double>* in,
+                                    std::complex<double>* out) {
+  check(cam_heff_apply(comp->handle(), reinterpret_cast<const double*>(in), reinterpret_cast<double*>(out),
+                       CAM_MEM_HOST, CAM_FP64));
+}
+
+/* MakeOddBondHamiltonian / MakeEvenBondHamiltonian / MakeBondEvolutionOperator
+ * (TEBD_routines.cpp:826-880). */
+inline MPX_matrix MakeOddBondHamiltonian(const MPX_matrix& H) {
+  cam_tensor* t = nullptr;
+  check(cam_bond_hamiltonian(Context::get(), H.handle(), 1, &t));
+  return MPX_matrix(t);
+}
+inline MPX_matrix MakeEvenBondHamiltonian(const MPX_matrix& H) {
+  cam_tensor* t = nullptr;
+  check(cam_bond_hamiltonian(Context::get(), H.handle(), 0, &t));
+  return MPX_matrix(t);
+}
+inline MPX_matrix MakeBondEvolutionOperator(const MPX_matrix& BondH, double timestep) {
+  cam_tensor* t = nullptr;
+  check(cam_bond_evolution_operator(Context::get(), BondH.handle(), timestep, &t));
+  return MPX_matrix(t);
+}
+
+/* iTEBD (TEBD_routines.hpp): unit cell {A0, A1, Lambdas[0], Lambdas[1]}. */
+class iTEBD {
+ public:
+  iTEBD(const MPX_matrix& H, const MPX_matrix& A0, const MPX_matrix& A1, const std::vector<double>& lambda0,
+        const std::vector<double>& lambda1, double time_step_size, int order = 2) {
+    cam_itebd* it = nullptr;
+    check(cam_itebd_create(Context::get(), H.handle(), A0.handle(), A1.handle(), lambda0.data(),
+                           static_cast<int64_t>(lambda0.size()), lambda1.data(),
+                           static_cast<int64_t>(lambda1.size()), time_step_size, order, &it));
+    it_ = std::shared_ptr<cam_itebd>(it, [](cam_itebd* p) {
+      if (p) cam_itebd_destroy(p);
+    });
+  }
+  /* iTEBD::evolve without measurement steps; returns {steps, time,
+   * truncation, entropy, chi, seconds, gate flops}. */
+  std::vector<double> evolve(MPXInt num_steps, MPXInt bond_dimension, double minS) {
+    std::vector<double> r(7);
+    check(cam_itebd_evolve(it_.get(), num_steps, bond_dimension, minS, r.data()));
+    return r;
+  }
+  std::vector<double> Lambda(int k) const {
+    int64_t n = 0;
+    check(cam_itebd_lambda(it_.get(), k, nullptr, 0, &n));
+    std::vector<double> v(static_cast<size_t>(std::max<int64_t>(n, 1)));
+    check(cam_itebd_lambda(it_.get(), k, v.data(), static_cast<int64_t>(v.size()), &n));
+    v.resize(static_cast<size_t>(n));
+    return v;
+  }
+  MPX_matrix Matrix(int k) const {
+    cam_tensor* t = nullptr;
+    check(cam_itebd_tensor(it_.get(), k, &t));
+    return MPX_matrix(t);
+  }
+
+ private:
+  std::shared_ptr<cam_itebd> it_;
+};
+
+/* iDMRG growth (iDMRG::run / SuperBlock::grow_two_vertex). */
+struct iDMRGStep {
+  MPXInt N = 0, chi = 0, n = 0;
+  double Energy = 0.0, EnergyPerVertex = 0.0, Entropy = 0.0, Truncation = 0.0, Seconds = 0.0;
+};
+class iDMRG {
+ public:
+  iDMRG(const MPX_matrix& H, const State& target, bool converge = false, bool mixed = false) {
+    std::vector<int16_t> t(target.begin(), target.end());
+    cam_idmrg* d = nullptr;
+    check(cam_idmrg_create(Context::get(), H.handle(), t.data(), converge, mixed, 20, &d));
+    d_ = std::shared_ptr<cam_idmrg>(d, [](cam_idmrg* p) {
+      if (p) cam_idmrg_destroy(p);
+    });
+  }
+  iDMRGStep grow(MPXInt chi, double truncation = 0.0) {
+    double r[13];
+    check(cam_idmrg_grow(d_.get(), chi, truncation, r));
+    iDMRGStep s;
+    s.N = static_cast<MPXInt>(r[0]);
+    s.Energy = r[1];
+    s.EnergyPerVertex = r[2];
+    s.Entropy = r[3];
+    s.Truncation = r[4];
+    s.chi = static_cast<MPXInt>(r[5]);
+    s.n = static_cast<MPXInt>(r[6]);
+    s.Seconds = r[10];
+    return s;
+  }
+
+ private:
+  std::shared_ptr<cam_idmrg> d_;
+};
+
+}  // namespace cam_ajaj
+
+#endif

@@ -1,0 +1,102 @@
+// The reference's spin_half_example (examples/user_defined/spin_half_example,
+// README.md:31-40) through the C++ layer that mirrors the reference API:
+// H = -sum (sx sx + sy sy + sz sz) + 1e-5 sum sz on a spin-1/2 chain with
+// Pauli operators, U(1) charge = 2 Sz, iDMRG growth with chi = 50 and the
+// reference's growth-step eigensolver setting.  The README's growth
+// energies -1, -3, -5, -7, -9 and entanglement entropies must come out.
+// Also exercises contract / reorder / SVD / TwoVertexComponents::HED.
+//
+// Exit status 0 on success; prints one line per check.
+#define CAM_AJAJ_THROW
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+#include "chainamps_amd.hpp"
+
+using namespace cam_ajaj;
+typedef std::complex<double> C;
+
+static int failures = 0;
+static void expect(bool ok, const char* what, double got, double want) {
+  std::printf("%s %s: got %.15g want %.15g\n", ok ? "ok  " : "FAIL", what, got, want);
+  if (!ok) ++failures;
+}
+
+int main() {
+  const std::vector<short> rules{0};   // one U(1) charge
+  const StateArray phys{{1}, {-1}};    // up, down
+  // sigma^+ (down -> up), sigma^- (up -> down), sigma^z, identity; d = 2
+  auto op = [](int which, int r, int c) -> C {
+    switch (which) {
+      case 0: return (r == c) ? 1.0 : 0.0;                   // I
+      case 1: return (r == 0 && c == 1) ? 1.0 : 0.0;         // s+
+      case 2: return (r == 1 && c == 0) ? 1.0 : 0.0;         // s-
+      default: return (r == c) ? (r == 0 ? 1.0 : -1.0) : 0.0;  // sz
+    }
+  };
+  // lower-triangular MPO, w = 5: W[0,0] = I, W[4,4] = I, W[4,0] = hz sz,
+  // bond terms J A_i B_{i+1}: W[4,j] = J A, W[j,0] = B with
+  // (A, B, J) = (s+, s-, -2), (s-, s+, -2), (sz, sz, -1)
+  const int d = 2, w = 5;
+  const StateArray bq{{0}, {2}, {-2}, {0}, {0}};
+  std::vector<C> Wd(static_cast<size_t>(d * w * d * w), 0.0);
+  auto at = [&](int sp, int bl, int s, int br) -> C& { return Wd[sp + d * (bl + w * (s + d * br))]; };
+  const int A[3] = {1, 2, 3}, B[3] = {2, 1, 3};
+  const double J[3] = {-2.0, -2.0, -1.0};
+  for (int sp = 0; sp < d; ++sp)
+    for (int s = 0; s < d; ++s) {
+      at(sp, 0, s, 0) = op(0, sp, s);
+      at(sp, w - 1, s, w - 1) = op(0, sp, s);
+      at(sp, w - 1, s, 0) = 1e-5 * op(3, sp, s);
+      for (int c = 0; c < 3; ++c) {
+        at(sp, w - 1, s, 1 + c) = J[c] * op(A[c], sp, s);
+        at(sp, 1 + c, s, 0) = op(B[c], sp, s);
+      }
+    }
+  const std::vector<MPXIndex> widx{MPXIndex(true, phys, true), MPXIndex(true, bq), MPXIndex(false, phys, true),
+                                   MPXIndex(false, bq)};
+  const MPX_matrix H = MPX_matrix::FromDense(rules, widx, 2, Wd);
+
+  // growth energies and entropies of README.md:31-40
+  const double energy[5] = {-1.0, -3.0, -5.0, -7.0, -9.0};
+  const double entropy[5] = {0.6931471805599454, 0.8675632284814612, 1.018230153951394, 1.138073514962318,
+                             1.235866276237509};
+  iDMRG dmrg(H, State{0}, /*converge=*/false);
+  for (int step = 0; step < 5; ++step) {
+    const iDMRGStep r = dmrg.grow(50);
+    char what[64];
+    std::snprintf(what, sizeof what, "iDMRG N=%lld energy", r.N);
+    expect(std::fabs(r.Energy - energy[step]) < 1e-10, what, r.Energy, energy[step]);
+    std::snprintf(what, sizeof what, "iDMRG N=%lld entropy", r.N);
+    expect(std::fabs(r.Entropy - entropy[step]) < 1e-9, what, r.Entropy, entropy[step]);
+  }
+
+  // contract / reorder / SVD on a two-site product: M = a (x) b over a
+  // shared bond with known singular values
+  const StateArray one{{0}};
+  const StateArray bond{{0}, {0}, {0}};
+  std::vector<C> a(6), b(6);
+  for (int i = 0; i < 6; ++i) {
+    a[i] = C(0.1 * (i + 1), 0.05 * i);
+    b[i] = C(0.2 - 0.03 * i, -0.01 * i);
+  }
+  const MPX_matrix Ma = MPX_matrix::FromDense(rules, {MPXIndex(true, bond), MPXIndex(false, {{0}, {0}})}, 1, a);
+  const MPX_matrix Mb = MPX_matrix::FromDense(rules, {MPXIndex(true, {{0}, {0}}), MPXIndex(false, bond)}, 1, b);
+  const MPX_matrix AB = contract(Ma, false, Mb, false, MPXPair(1, 0));   // 3 x 3, rank 2
+  const MPXDecomposition dec = reorder(AB, false, {0, 1}, 1).SVD();
+  expect(dec.Values.size() == 2 || (dec.Values.size() == 3 && dec.Values[2] < 1e-14), "SVD rank of a (x) b",
+         static_cast<double>(dec.Values.size()), 2.0);
+  double fro2 = 0.0;
+  std::vector<long long> p, i;
+  std::vector<C> x;
+  AB.to_csc(p, i, x);
+  for (const C& v : x) fro2 += std::norm(v);
+  double s2 = 0.0;
+  for (double v : dec.Values) s2 += v * v;
+  expect(std::fabs(s2 - fro2) < 1e-13 * fro2, "SVD sum s^2 = |AB|_F^2", s2, fro2);
+
+  std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
+  return failures ? 1 : 0;
+}
