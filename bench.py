@@ -111,6 +111,31 @@ def optimal_order_flops_dense(LB, W, RB, comp):
     return flops
 
 
+def idmrg_steps(ctx, cam, workloads, a):
+    """Times cam.IDMRG growth steps on the d-level coupled-chain MPO with
+    bond dimension cap chi, FP64 and FP32-mixed; the last step is the one at
+    full size (n = chi^2 d^2 / 2 for two parity sectors)."""
+    W, _, _, _ = workloads.coupled_chain_mpo(ctx, d=a.d, seed=0, Jperp=0.1)
+    out = {}
+    for mixed, name in ((False, "fp64"), (True, "mixed")):
+        dm = cam.IDMRG(W, converge=False, mixed=mixed)
+        steps = []
+        for _ in range(a.idmrg_steps):
+            r = dm.grow(a.chi)
+            steps.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+                          if k in ("N", "chi", "n", "energy", "truncation", "seconds", "eig_seconds",
+                                   "svd_seconds", "block_seconds", "matvecs_fp64", "matvecs_fp32")})
+            print(f"# idmrg {name} step N={r['N']} chi={r['chi']} n={r['n']} {r['seconds']:.2f} s", file=sys.stderr,
+                  flush=True)
+        dm.close()
+        out[name] = steps
+    out["last_step_seconds"] = {k: out[k][-1]["seconds"] for k in ("fp64", "mixed")}
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"][-1]["seconds"] / out["mixed"][-1]["seconds"], 3)
+    out["note"] = ("growth steps with the reference's growth-step eigensolver setting (tol 0.1, ncv 20, "
+                   "predicted start vector); seconds = eigensolve + SVD + block updates")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +145,9 @@ def main():
     ap.add_argument("--d", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true", help="skip the growth-step eigensolve timing")
+    ap.add_argument("--no-idmrg", action="store_true", help="skip the iDMRG step timing")
+    ap.add_argument("--idmrg-steps", type=int, default=4,
+                    help="growth steps per precision (the bond reaches chi at step 3 for d=64)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -186,6 +214,15 @@ def main():
         eig["speedup_mixed_vs_fp64"] = round(eig["fp64"]["seconds"] / eig["mixed"]["seconds"], 3)
         eig["note"] = ("Lanczos eigensolve with the growth-step setting; the SVD and the block updates of "
                        "the step are not included")
+    launch = {"fp64": H.get_launch(cam.CAM_FP64), "fp32": H.get_launch(cam.CAM_FP32)}
+    # whole iDMRG growth steps (SuperBlock::grow_two_vertex, DMRG_routines.cpp:309-374)
+    # of the same model MPO: block updates, H_eff build, eigensolve from the
+    # predicted guess (growth setting), SVD truncation to chi and prediction,
+    # FP64 against FP32-mixed; the matvec buffers above are released first
+    del x, y
+    H.close()
+    torch.cuda.empty_cache()
+    idmrg = None if a.no_idmrg else idmrg_steps(ctx, cam, workloads, a)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_budget)
     line = {
         "metric": "H_eff matvec TFLOP/s + iDMRG step time, chi=2000 d=64, FP32-mixed vs FP64",
@@ -213,8 +250,9 @@ def main():
                      "kernel": f"grouped_gemm_kernel<double> stage {stage_names[dom]}"},
         "cpu_baseline": cpu,
         "growth_step_eigensolve": eig or None,
-        "autotuned_launch": {"fields": "(algo 0=4M/1=3M, tile_mode, wg_per_cu) per stage L, W1, W2, R",
-                             "fp64": H.get_launch(cam.CAM_FP64), "fp32": H.get_launch(cam.CAM_FP32)},
+        "idmrg_step": idmrg,
+        "autotuned_launch": {"fields": "(algo 0=4M/1=3M, tile_mode, wg_per_cu, kchunk) per stage L, W1, W2, R",
+                             **launch},
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(line))

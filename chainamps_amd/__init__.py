@@ -6,10 +6,10 @@ package only binds that ABI with ctypes; importing it never falls back to a
 CPU implementation -- a missing library is an ImportError-like failure at
 first use (``load_library``).
 """
-from .capi import (CamError, Context, Tensor, Heff, IDMRG, ITEBD, contract, reorder, svd_truncate,
+from .capi import (CamError, Context, Tensor, Heff, IDMRG, ITEBD, FTEBD, contract, reorder, svd_truncate,
                    tebd_gate, extract_mpo_block, bond_hamiltonian, bond_evolution_operator,
                    load_library, library_path, CAM_FP64, CAM_FP32)
 
-__all__ = ["CamError", "Context", "Tensor", "Heff", "IDMRG", "ITEBD", "contract", "reorder", "svd_truncate",
+__all__ = ["CamError", "Context", "Tensor", "Heff", "IDMRG", "ITEBD", "FTEBD", "contract", "reorder", "svd_truncate",
            "tebd_gate", "extract_mpo_block", "bond_hamiltonian", "bond_evolution_operator",
            "load_library", "library_path", "CAM_FP64", "CAM_FP32"]

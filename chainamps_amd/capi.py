@@ -72,7 +72,7 @@ _SIGS = {
     "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
-    "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int],
+    "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
     "cam_heff_to_blocked": [_p, _p, _p],
     "cam_heff_from_blocked": [_p, _p, _p],
@@ -313,15 +313,15 @@ class Heff:
         _check(_LIB.cam_heff_profile(self.h, precision, reps, _ptr(ms, C.c_double), _ptr(fl, C.c_double)))
         return ms, fl
 
-    def set_launch(self, precision=CAM_FP64, algo=1, tile_mode=1, wg_per_cu=4):
+    def set_launch(self, precision=CAM_FP64, algo=1, tile_mode=1, wg_per_cu=4, kchunk=16):
         """Fix the stage GEMM launch configuration (disables autotuning)."""
-        _check(_LIB.cam_heff_set_launch(self.h, precision, algo, tile_mode, wg_per_cu))
+        _check(_LIB.cam_heff_set_launch(self.h, precision, algo, tile_mode, wg_per_cu, kchunk))
 
     def get_launch(self, precision=CAM_FP64):
-        """[(algo, tile_mode, wg_per_cu)] for the stages L, W, W, R."""
-        cfg = np.zeros(12, np.int32)
+        """[(algo, tile_mode, wg_per_cu, kchunk)] for the stages L, W, W, R."""
+        cfg = np.zeros(16, np.int32)
         _check(_LIB.cam_heff_get_launch(self.h, precision, _ptr(cfg, C.c_int)))
-        return [tuple(int(v) for v in cfg[3 * i:3 * i + 3]) for i in range(4)]
+        return [tuple(int(v) for v in cfg[4 * i:4 * i + 4]) for i in range(4)]
 
     def to_blocked(self, din, dout):
         _check(_LIB.cam_heff_to_blocked(self.h, din, dout))
@@ -437,7 +437,8 @@ _SIGS_IDMRG = {
 _SIGS.update(_SIGS_IDMRG)
 
 IDMRG_FIELDS = ("N", "energy", "energy_per_vertex", "entropy", "truncation", "chi", "n",
-                "matvecs_fp64", "matvecs_fp32", "residual", "seconds", "heff_flops")
+                "matvecs_fp64", "matvecs_fp32", "residual", "seconds", "heff_flops",
+                "eig_seconds", "svd_seconds", "block_seconds")
 
 
 class IDMRG:
@@ -454,7 +455,7 @@ class IDMRG:
         self.h = h
 
     def grow(self, chi, truncation=0.0):
-        rec = np.zeros(13)
+        rec = np.zeros(16)
         _check(_LIB.cam_idmrg_grow(self.h, int(chi), float(truncation), _ptr(rec, C.c_double)))
         d = dict(zip(IDMRG_FIELDS, rec.tolist()))
         for k in ("N", "chi", "n", "matvecs_fp64", "matvecs_fp32"):
@@ -490,6 +491,16 @@ _SIGS_ITEBD = {
 _SIGS.update(_SIGS_ITEBD)
 
 ITEBD_FIELDS = ("steps", "time", "truncation", "entropy", "chi", "seconds", "gate_flops")
+
+_SIGS_FTEBD = {
+    "cam_ftebd_create": [_p, _p, C.POINTER(_p), C.c_int64, C.c_double, C.c_int, C.POINTER(_p)],
+    "cam_ftebd_evolve": [_p, C.c_int64, C.c_int64, C.c_double, C.c_int64],
+    "cam_ftebd_records": [_p, _dp, C.c_int64, _i64p],
+    "cam_ftebd_site": [_p, C.c_int64, C.POINTER(_p)],
+    "cam_ftebd_destroy": [_p],
+}
+_SIGS.update(_SIGS_FTEBD)
+FTEBD_FIELDS = ("step", "time", "truncation", "entropy", "overlap_re", "overlap_im", "abs_overlap")
 
 
 class ITEBD:
@@ -527,6 +538,52 @@ class ITEBD:
     def close(self):
         if getattr(self, "h", None):
             _LIB.cam_itebd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FTEBD:
+    """Device finite-chain TEBD (TEBD::evolve with apply_to_odd/even_bonds and
+    left_canonise, Trotter orders 1, 2, 4) on a left-canonical MPS given as
+    a list of site tensors [s, a_l (in), a_r (out)]."""
+
+    def __init__(self, W: Tensor, sites, dt, order=2):
+        self.ctx = W.ctx
+        arr = (_p * len(sites))(*[t.h for t in sites])
+        h = _p()
+        _check(_LIB.cam_ftebd_create(W.ctx.h, W.h, arr, len(sites), float(dt), int(order), C.byref(h)))
+        self.h = h
+        self.N = len(sites)
+
+    def evolve(self, steps, D=0, minS=0.0, interval=1):
+        _check(_LIB.cam_ftebd_evolve(self.h, int(steps), int(D), float(minS), int(interval)))
+
+    def records(self):
+        n = C.c_int64()
+        _check(_LIB.cam_ftebd_records(self.h, None, 0, C.byref(n)))
+        v = np.zeros(max(7 * n.value, 1))
+        _check(_LIB.cam_ftebd_records(self.h, _ptr(v, C.c_double), n.value, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            d = dict(zip(FTEBD_FIELDS, v[7 * i:7 * i + 7].tolist()))
+            d["step"] = int(d["step"])
+            d["overlap"] = complex(d.pop("overlap_re"), d.pop("overlap_im"))
+            out.append(d)
+        return out
+
+    def site(self, v) -> Tensor:
+        h = _p()
+        _check(_LIB.cam_ftebd_site(self.h, int(v), C.byref(h)))
+        return Tensor(self.ctx, h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_ftebd_destroy(self.h)
             self.h = None
 
     def __del__(self):

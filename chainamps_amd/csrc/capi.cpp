@@ -9,6 +9,7 @@
 #include "solvers.hpp"
 #include "idmrg.hpp"
 #include "itebd.hpp"
+#include "ftebd.hpp"
 #include <memory>
 
 using namespace cam;
@@ -277,17 +278,19 @@ int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, dou
   CAM_API_END
 }
 
-int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu) {
+int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu, int kchunk) {
   CAM_API_BEGIN
   set_device(h->owner);
   CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
   CAM_REQUIRE(algo == kAlgo4M || algo == kAlgo3M, "algo must be 0 (four real products) or 1 (three)");
   CAM_REQUIRE(tile_mode >= 0 && tile_mode < kTileModes, "tile_mode out of range");
   CAM_REQUIRE(wg_per_cu >= 1 && wg_per_cu <= 64, "wg_per_cu out of range");
+  CAM_REQUIRE(kchunk == 8 || kchunk == 16, "kchunk must be 8 or 16");
   LaunchCfg c;
   c.algo = algo;
   c.tile_mode = tile_mode;
   c.wg_per_cu = wg_per_cu;
+  c.kc = kchunk;
   h->h.set_launch(precision, c);
   CAM_API_END
 }
@@ -297,9 +300,10 @@ int cam_heff_get_launch(const cam_heff* h, int precision, int* cfg) {
   CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
   for (int i = 0; i < 4; ++i) {
     const LaunchCfg& c = h->h.st[i].cfg[precision];
-    cfg[3 * i] = c.algo;
-    cfg[3 * i + 1] = c.tile_mode;
-    cfg[3 * i + 2] = c.wg_per_cu;
+    cfg[4 * i] = c.algo;
+    cfg[4 * i + 1] = c.tile_mode;
+    cfg[4 * i + 2] = c.wg_per_cu;
+    cfg[4 * i + 3] = c.kc;
   }
   CAM_API_END
 }
@@ -425,6 +429,10 @@ struct cam_itebd_s {
   std::unique_ptr<cam::ITEBD> d;
   cam_ctx* owner;
 };
+struct cam_ftebd_s {
+  std::unique_ptr<cam::FTEBD> d;
+  cam_ctx* owner;
+};
 
 extern "C" {
 
@@ -444,9 +452,11 @@ int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record)
   CAM_API_BEGIN
   set_device(d->owner);
   const cam::IDMRGRecord r = d->d->grow(chi, truncation);
-  const double v[13] = {double(r.N), r.energy, r.energy_per_vertex, r.entropy, r.truncation, double(r.chi),
-                        double(r.n), r.matvecs_fp64, r.matvecs_fp32, r.residual, r.seconds, r.heff_flops, 0.0};
-  std::copy(v, v + 13, record);
+  const double v[16] = {double(r.N),   r.energy,       r.energy_per_vertex, r.entropy,
+                        r.truncation,  double(r.chi),  double(r.n),         r.matvecs_fp64,
+                        r.matvecs_fp32, r.residual,    r.seconds,           r.heff_flops,
+                        r.eig_seconds, r.svd_seconds,  r.block_seconds,     0.0};
+  std::copy(v, v + 16, record);
   CAM_API_END
 }
 
@@ -517,6 +527,66 @@ int cam_itebd_tensor(const cam_itebd* it, int k, cam_tensor** out) {
 int cam_itebd_destroy(cam_itebd* it) {
   CAM_API_BEGIN
   delete it;
+  CAM_API_END
+}
+
+int cam_ftebd_create(cam_ctx* c, const cam_tensor* W, const cam_tensor* const* sites, int64_t nsites, double dt,
+                     int order, cam_ftebd** out) {
+  CAM_API_BEGIN
+  set_device(c);
+  CAM_REQUIRE(sites && nsites >= 2, "finite TEBD: at least two site tensors");
+  std::vector<TensorPtr> v;
+  for (int64_t i = 0; i < nsites; ++i) {
+    CAM_REQUIRE(sites[i], "finite TEBD: null site tensor");
+    v.push_back(sites[i]->t);
+  }
+  auto* r = new cam_ftebd;
+  r->owner = c;
+  try {
+    r->d.reset(new cam::FTEBD(c->ctx, *W->t, v, dt, order));
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  *out = r;
+  CAM_API_END
+}
+
+int cam_ftebd_evolve(cam_ftebd* t, int64_t nsteps, int64_t D, double minS, int64_t interval) {
+  CAM_API_BEGIN
+  set_device(t->owner);
+  t->d->evolve(nsteps, D, minS, interval);
+  CAM_API_END
+}
+
+int cam_ftebd_records(const cam_ftebd* t, double* records, int64_t cap, int64_t* n) {
+  CAM_API_BEGIN
+  const auto& R = t->d->records;
+  if (n) *n = static_cast<int64_t>(R.size());
+  CAM_REQUIRE(static_cast<int64_t>(R.size()) <= cap || !records, "capacity too small");
+  if (records)
+    for (size_t i = 0; i < R.size(); ++i) {
+      const double v[7] = {double(R[i].step), R[i].time,       R[i].truncation, R[i].entropy,
+                           R[i].overlap_re,   R[i].overlap_im, R[i].abs_overlap};
+      std::copy(v, v + 7, records + 7 * i);
+    }
+  CAM_API_END
+}
+
+int cam_ftebd_site(const cam_ftebd* t, int64_t v, cam_tensor** out) {
+  CAM_API_BEGIN
+  set_device(t->owner);
+  CAM_REQUIRE(v >= 1 && v <= t->d->N, "finite TEBD: site index out of range (1..N)");
+  const cam::BSTensor& A = *t->d->left[v];
+  std::vector<int> ident(A.nidx());
+  for (int i = 0; i < A.nidx(); ++i) ident[i] = i;
+  *out = new cam_tensor{reorder(t->owner->ctx, A, false, ident, A.numrows), t->owner};
+  CAM_API_END
+}
+
+int cam_ftebd_destroy(cam_ftebd* t) {
+  CAM_API_BEGIN
+  delete t;
   CAM_API_END
 }
 

@@ -11,8 +11,8 @@
 //     Exponentiate (sparse_interface.cpp:1707-1714) over the blockwise
 //     Hermitian decomposition SparseMatrix::HED (sparse_interface.cpp:
 //     1414-1441): per group of columns with equal outgoing charge, the
-//     full square block (zero eigenvalues kept) is diagonalised (rocSOLVER
-//     zheev), eigenvector entries |v| <= eps/rows are dropped, and
+//     full square block (zero eigenvalues kept) is diagonalised
+//     (hermitian_eig), eigenvector entries |v| <= eps/rows are dropped, and
 //     G = V diag(exp(-i tau lambda)) V^dagger (rocBLAS zgemm).  The result
 //     keeps the first four indices of the bond Hamiltonian.
 #include <rocblas/rocblas.h>
@@ -181,25 +181,18 @@ TensorPtr bond_evolution_operator(Ctx& ctx, const BSTensor& H, double timestep) 
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   int64_t gmax = 1;
   for (int g = 0; g < ng; ++g) gmax = std::max(gmax, gsize[g]);
-  DevBuf<double> lam(gmax), off(gmax);
-  DevBuf<int> dinfo(1);
+  DevBuf<double> lam(gmax);
   for (int g = 0; g < ng; ++g) {
     const int64_t m = gsize[g];
     auto* Ag = reinterpret_cast<rocblas_double_complex*>(A.get() + goff[g]);
-    rocblas_status st = rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(m), Ag,
-                                         static_cast<rocblas_int>(m), lam.get(), off.get(), dinfo.get());
-    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheev failed");
-    int info = 0;
-    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    CAM_REQUIRE(info == 0, "bond Hamiltonian eigensolver did not converge");
+    hermitian_eig(ctx, reinterpret_cast<cplx*>(Ag), m, lam.get());
     // exp(factor * lambda) with factor = (0, -timestep)
     hipLaunchKernelGGL(exp_columns_kernel, dim3(grid_of(m * m)), dim3(256), 0, s,
                        reinterpret_cast<double2*>(A.get() + goff[g]), reinterpret_cast<double2*>(B.get() + goff[g]),
                        lam.get(), m, 0.0, -timestep, eps / static_cast<double>(m));
     HIP_CHECK(hipGetLastError());
     const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
-    st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose, static_cast<rocblas_int>(m),
+    const rocblas_status st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose, static_cast<rocblas_int>(m),
                        static_cast<rocblas_int>(m), static_cast<rocblas_int>(m), &one,
                        reinterpret_cast<const rocblas_double_complex*>(B.get() + goff[g]), static_cast<rocblas_int>(m),
                        Ag, static_cast<rocblas_int>(m), &zero,

@@ -2,8 +2,8 @@
 //
 // Workgroup = 256 threads = 4 wave64s arranged 2x2; a TM x TN output tile
 // per workgroup, each wave owning a (TM/2) x (TN/2) sub-tile made of 16x16
-// MFMA fragments.  K is consumed in chunks of 16 staged through LDS with
-// real and imaginary parts in separate planes (so one MFMA operand is one
+// MFMA fragments.  K is consumed in chunks of KC (16 or 8) staged through LDS
+// with real and imaginary parts in separate planes (so one MFMA operand is one
 // LDS read); the next chunk is prefetched into registers while the current
 // one is multiplied.  A complex product is four real MFMAs,
 //   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre,
@@ -29,7 +29,7 @@ namespace cam {
 
 namespace {
 
-constexpr int KC = 16;     // K chunk staged in LDS
+constexpr int KCmax = 16;  // largest K chunk staged in LDS
 constexpr int kXcds = 8;   // MI355X accelerator complex dies (one L2 each)
 constexpr int PADL = 4;    // LDS row padding (elements)
 
@@ -82,8 +82,12 @@ template <> struct Mf<float> {
 // G3: complex products with three real MFMAs (P1 = Ar Br, P2 = Ai Bi,
 // P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2) instead of four;
 // the error stays O(eps |A||B|) per product, as for the four-product form.
-template <typename R, int TM, int TN, bool G3>
-__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __restrict__ probs,
+// KC = 8 halves the LDS footprint and the prefetch registers: with the
+// register budget held to 168 (three waves per SIMD) three 64x64 FP64
+// workgroups fit on a CU instead of two (the 128-wide tiles keep their
+// registers: held to 168 they spill)
+template <typename R, int TM, int TN, bool G3, int KC>
+__global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void grouped_gemm_kernel(const GemmProb* __restrict__ probs,
                                                            const GemmSeg* __restrict__ segs,
                                                            const GemmTile* __restrict__ tiles,
                                                            int ntiles, GemmBases bases) {
@@ -349,12 +353,16 @@ void launch_shape(const GemmPlan& plan, const LaunchCfg& c, int shape, hipStream
   int grid = std::min(nt, cus * std::max(1, c.wg_per_cu));
   if (grid_limit > 0) grid = std::min(grid, grid_limit);
   const GemmTile* tiles = plan.d_tiles[c.tile_mode][shape].get();
-  if (c.algo == kAlgo3M)
-    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, true>), dim3(grid), dim3(256), 0, s,
-                       plan.d_probs.get(), plan.d_segs.get(), tiles, nt, bases);
+  const GemmProb* P = plan.d_probs.get();
+  const GemmSeg* S = plan.d_segs.get();
+  if (c.algo == kAlgo3M && c.kc == 8)
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, true, 8>), dim3(grid), dim3(256), 0, s, P, S, tiles, nt, bases);
+  else if (c.algo == kAlgo3M)
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, true, 16>), dim3(grid), dim3(256), 0, s, P, S, tiles, nt, bases);
+  else if (c.kc == 8)
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, false, 8>), dim3(grid), dim3(256), 0, s, P, S, tiles, nt, bases);
   else
-    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, false>), dim3(grid), dim3(256), 0, s,
-                       plan.d_probs.get(), plan.d_segs.get(), tiles, nt, bases);
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, TM, TN, false, 16>), dim3(grid), dim3(256), 0, s, P, S, tiles, nt, bases);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -440,7 +448,7 @@ void GemmPlan::build_tiles_mode(int mode) {
       }
     }
     int64_t ktot = 0;
-    for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) ktot += (segs[s].K + KC - 1) / KC * KC;
+    for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) ktot += (segs[s].K + KCmax - 1) / KCmax * KCmax;
     if (ktot == 0 && p.beta) continue;   // nothing to add
     const int tm = kTileShapes[shape].tm, tn = kTileShapes[shape].tn;
     for (int b = 0; b < p.batch; ++b)
@@ -564,7 +572,8 @@ bool autotune(GemmPlan& plan, int precision, hipStream_t s, const GemmBases* bas
     }
     return best;
   };
-  // coordinate descent: product form, then tile policy, then grid size
+  // coordinate descent: product form, then tile policy, then K chunk, then
+  // grid size
   LaunchCfg cur = plan.cfg[precision];
   float tcur = time_it(cur);
   auto try_axis = [&](int LaunchCfg::*field, std::initializer_list<int> vals) {
@@ -578,6 +587,7 @@ bool autotune(GemmPlan& plan, int precision, hipStream_t s, const GemmBases* bas
   };
   try_axis(&LaunchCfg::algo, {kAlgo3M, kAlgo4M});
   try_axis(&LaunchCfg::tile_mode, {1, 0, 2, 3});
+  try_axis(&LaunchCfg::kc, {16, 8});
   try_axis(&LaunchCfg::wg_per_cu, {4, 2, 8});
   apply_choice(plan, precision, cur, s);
   // leave the output of the chosen configuration (every candidate computes

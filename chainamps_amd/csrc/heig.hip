@@ -1,0 +1,203 @@
+// Dense Hermitian eigendecomposition and the singular value decomposition
+// built on it, on the device.
+//
+// hermitian_eig: rocSOLVER divide and conquer (zheevd, tens of ms at
+// n = 1000-2000 where the QR iteration zheev takes seconds), accepted only
+// after a check of every eigenpair: |A v_j - w_j v_j| <= 1e-10 max|w| and
+// | |v_j| - 1 | <= 1e-10 (one GEMM); a decomposition that fails the check is
+// recomputed with zheev from a saved copy.  Both read the upper triangle,
+// and the check is made against the Hermitian matrix it defines.
+//
+// jordan_wielandt_svd: the singular triplets of an m x n matrix B from the
+// eigenpairs of the Hermitian (m+n) x (m+n) matrix [[0, B], [B^H, 0]],
+// whose eigenvalues are +-s_j (and m-n zeros) with eigenvectors
+// (u_j, +-v_j)/sqrt(2).  Unlike the eigenpairs of B^H B it does not square
+// the condition: each s_j is accurate to a few eps |B|, as a bidiagonal SVD.
+// For s_j > 0 the halves of an eigenvector have equal norms (from
+// B v = s u and B^H u = s v), and the mirror pair (u, -v) of eigenvalue -s_j
+// makes the u parts of distinct positive eigenvalues orthogonal, so the
+// extracted U and V are orthonormal; only values at the eps |B| level lose
+// their vectors, which is why the callers validate the triplets they keep.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "solvers.hpp"
+
+namespace cam {
+
+namespace {
+
+int grid_of(int64_t n) { return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256))); }
+
+rocblas_double_complex* Z(cplx* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
+
+// per column j: r[j] = |R_j - w_j V_j|^2, q[j] = |V_j|^2 (one workgroup per column)
+__global__ void eig_residual_kernel(const double2* __restrict__ R, const double2* __restrict__ V,
+                                    const double* __restrict__ w, int64_t n, double* __restrict__ r,
+                                    double* __restrict__ q) {
+  const int64_t j = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double2 x = R[i + j * n], v = V[i + j * n];
+    const double dx = x.x - w[j] * v.x, dy = x.y - w[j] * v.y;
+    a += dx * dx + dy * dy;
+    b += v.x * v.x + v.y * v.y;
+  }
+  __shared__ double sa[256], sb[256];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o];
+      sb[threadIdx.x] += sb[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    r[j] = sa[0];
+    q[j] = sb[0];
+  }
+}
+
+// H = [[0, B], [B^H, 0]] (N = m + n, column-major)
+__global__ void jordan_wielandt_kernel(const double2* __restrict__ B, int64_t ldb, int64_t m, int64_t n,
+                                       double2* __restrict__ H) {
+  const int64_t N = m + n, tot = N * N;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = e % N, c = e / N;
+    double2 v = make_double2(0.0, 0.0);
+    if (r < m && c >= m) v = B[r + (c - m) * ldb];
+    else if (r >= m && c < m) {
+      const double2 b = B[c + (r - m) * ldb];
+      v = make_double2(b.x, -b.y);
+    }
+    H[e] = v;
+  }
+}
+
+// U (m x k) = sqrt(2) X[0:m, N-1-j], Vh (k x n) = sqrt(2) conj(X[m:N, N-1-j])^T
+__global__ void jw_extract_kernel(const double2* __restrict__ X, int64_t m, int64_t n, int64_t k,
+                                  double2* __restrict__ U, double2* __restrict__ Vh) {
+  const int64_t N = m + n;
+  const double s2 = 1.4142135623730951;
+  const int64_t tu = m * k, tot = tu + k * n;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    if (e < tu) {
+      const int64_t i = e % m, j = e / m;
+      const double2 x = X[i + (N - 1 - j) * N];
+      U[e] = make_double2(s2 * x.x, s2 * x.y);
+    } else {
+      const int64_t f = e - tu;
+      const int64_t j = f % k, c = f / k;          // Vh(j, c)
+      const double2 x = X[m + c + (N - 1 - j) * N];
+      Vh[f] = make_double2(s2 * x.x, -s2 * x.y);
+    }
+  }
+}
+
+// A0 = the Hermitian matrix defined by the upper triangle of A (what the
+// eigensolvers read with fill_upper)
+__global__ void hermitian_from_upper_kernel(const double2* __restrict__ A, int64_t n, double2* __restrict__ A0) {
+  const int64_t tot = n * n;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = e % n, c = e / n;
+    if (r <= c) {
+      A0[e] = A[e];
+    } else {
+      const double2 v = A[c + r * n];
+      A0[e] = make_double2(v.x, -v.y);
+    }
+  }
+}
+
+bool check_eigenpairs(Ctx& ctx, const cplx* A0, const cplx* V, const double* w, int64_t n) {
+  hipStream_t s = ctx.stream;
+  rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
+  DevBuf<cplx> R(static_cast<size_t>(n) * n);
+  DevBuf<double> r(n), q(n);
+  const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+  const rocblas_status st = rocblas_zgemm(
+      h, rocblas_operation_none, rocblas_operation_none, static_cast<rocblas_int>(n), static_cast<rocblas_int>(n),
+      static_cast<rocblas_int>(n), &one, reinterpret_cast<const rocblas_double_complex*>(A0),
+      static_cast<rocblas_int>(n), reinterpret_cast<const rocblas_double_complex*>(V), static_cast<rocblas_int>(n),
+      &zero, Z(R.get()), static_cast<rocblas_int>(n));
+  CAM_REQUIRE(st == rocblas_status_success, "rocblas_zgemm failed");
+  hipLaunchKernelGGL(eig_residual_kernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(R.get()), reinterpret_cast<const double2*>(V), w, n, r.get(),
+                     q.get());
+  HIP_CHECK(hipGetLastError());
+  std::vector<double> hr(n), hq(n), hw(n);
+  HIP_CHECK(hipMemcpyAsync(hr.data(), r.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hq.data(), q.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hw.data(), w, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  double wmax = 0.0;
+  for (double x : hw) wmax = std::max(wmax, std::fabs(x));
+  const double tol = 1e-10 * std::max(wmax, 1e-300);
+  for (int64_t j = 0; j < n; ++j) {
+    if (!(std::fabs(std::sqrt(hq[j]) - 1.0) <= 1e-10)) return false;
+    if (!(std::sqrt(hr[j]) <= tol)) return false;
+    if (j && !(hw[j] >= hw[j - 1])) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+void hermitian_eig(Ctx& ctx, cplx* A, int64_t n, double* w) {
+  CAM_REQUIRE(n >= 1 && n < (int64_t(1) << 31), "hermitian_eig: size out of range");
+  hipStream_t s = ctx.stream;
+  rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
+  const size_t nn = static_cast<size_t>(n) * n;
+  DevBuf<cplx> A0(nn);
+  DevBuf<double> off(n);
+  DevBuf<int> dinfo(1);
+  hipLaunchKernelGGL(hermitian_from_upper_kernel, dim3(grid_of(n * n)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(A), n, reinterpret_cast<double2*>(A0.get()));
+  HIP_CHECK(hipGetLastError());
+  rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(n),
+                                       Z(A), static_cast<rocblas_int>(n), w, off.get(), dinfo.get());
+  CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheevd failed");
+  int info = 0;
+  HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (info == 0 && check_eigenpairs(ctx, A0.get(), A, w, n)) return;
+  // fallback: QR iteration from the saved matrix
+  HIP_CHECK(hipMemcpyAsync(A, A0.get(), nn * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+  st = rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_upper, static_cast<rocblas_int>(n), Z(A),
+                       static_cast<rocblas_int>(n), w, off.get(), dinfo.get());
+  CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zheev failed");
+  HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  CAM_REQUIRE(info == 0, "Hermitian eigensolver did not converge");
+  CAM_REQUIRE(check_eigenpairs(ctx, A0.get(), A, w, n), "Hermitian eigensolver: eigenpairs failed the check");
+}
+
+void jordan_wielandt_svd(Ctx& ctx, const cplx* B, int64_t ldb, int64_t m, int64_t n, int64_t k,
+                         std::vector<double>& S, cplx* U, cplx* Vh) {
+  CAM_REQUIRE(k >= 1 && k <= std::min(m, n), "jordan_wielandt_svd: k out of range");
+  hipStream_t s = ctx.stream;
+  const int64_t N = m + n;
+  DevBuf<cplx> H(static_cast<size_t>(N) * N);
+  DevBuf<double> w(N);
+  hipLaunchKernelGGL(jordan_wielandt_kernel, dim3(grid_of(N * N)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(B), ldb, m, n, reinterpret_cast<double2*>(H.get()));
+  HIP_CHECK(hipGetLastError());
+  hermitian_eig(ctx, H.get(), N, w.get());
+  hipLaunchKernelGGL(jw_extract_kernel, dim3(grid_of((m + n) * k)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(H.get()), m, n, k, reinterpret_cast<double2*>(U),
+                     reinterpret_cast<double2*>(Vh));
+  HIP_CHECK(hipGetLastError());
+  std::vector<double> hw(N);
+  HIP_CHECK(hipMemcpyAsync(hw.data(), w.get(), N * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  S.resize(k);
+  for (int64_t j = 0; j < k; ++j) S[j] = std::max(hw[N - 1 - j], 0.0);
+}
+
+}  // namespace cam

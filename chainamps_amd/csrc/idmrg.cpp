@@ -44,14 +44,14 @@ QIndex scalar_index(bool inward, const Charge& q) {
   return r;
 }
 
+}  // namespace
+
 double entropy(const std::vector<double>& v) {
   double s = 0.0;
   for (double x : v)
     if (x != 0.0) s -= x * x * std::log(x * x);
   return s;
 }
-
-}  // namespace
 
 TensorPtr make_dummy_left_block(Ctx& ctx, const BSTensor& W) {
   const Charge z = W.rules.zero();
@@ -130,6 +130,15 @@ IDMRG::IDMRG(Ctx& c, TensorPtr mpo, const Charge& tgt, bool conv, bool mix, int 
 
 IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
   const auto t0 = std::chrono::steady_clock::now();
+  // phase clock: the stream is drained so each phase is charged its own work
+  auto lap = [this](std::chrono::steady_clock::time_point& t) {
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto now = std::chrono::steady_clock::now();
+    const double dt = std::chrono::duration<double>(now - t).count();
+    t = now;
+    return dt;
+  };
+  auto tp = t0;
   IDMRGRecord rec{};
   Heff H;
   double info[4] = {0, 0, 0, 0};
@@ -138,13 +147,16 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
     LBd = make_dummy_left_block(*ctx, *W);
     RBd = make_dummy_right_block(*ctx, *W, target);
     H.build(*ctx, LBd, W, RBd);
-    DevBuf<cplx> x(std::max<int64_t>(H.n, 1));
-    E = eigs_lowest(H, x.get(), false, true, 0.0, ncv, false, info);
-    TensorPtr psi = H.tensor_from_vector(x.get());
+    xbuf.ensure(std::max<int64_t>(H.n, 1));
+    rec.block_seconds += lap(tp);
+    E = eigs_lowest(H, xbuf.get(), false, true, 0.0, ncv, false, info);
+    rec.eig_seconds = lap(tp);
+    TensorPtr psi = H.tensor_from_vector(xbuf.get());
     const int64_t bond = (chi < 1 && trunc <= 0.0) ? W->idx[0].size : chi;
     double disc = 0.0;
     svd_truncate(*ctx, *psi, bond, trunc, U, Vd, values, disc);
     truncation = disc;
+    rec.svd_seconds = lap(tp);
     LB = make_left_block(*ctx, *LBd, *W, *U);
     RB = make_right_block(*ctx, *RBd, *W, *Vd);
     size = 2;
@@ -157,17 +169,20 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
     size += 2;
     prev_lambda = values;
     H.build(*ctx, LB, W, RB);
-    DevBuf<cplx> x(std::max<int64_t>(H.n, 1));
+    xbuf.ensure(std::max<int64_t>(H.n, 1));
     bool have = false;
     if (guess) {
-      H.vector_from_tensor(*guess, x.get());
+      H.vector_from_tensor(*guess, xbuf.get());
       have = true;
     }
-    E = eigs_lowest(H, x.get(), have, converge, 0.0, ncv, mixed, info);
-    TensorPtr psi = H.tensor_from_vector(x.get());
+    rec.block_seconds += lap(tp);
+    E = eigs_lowest(H, xbuf.get(), have, converge, 0.0, ncv, mixed, info);
+    rec.eig_seconds = lap(tp);
+    TensorPtr psi = H.tensor_from_vector(xbuf.get());
     double disc = 0.0;
     svd_truncate(*ctx, *psi, chi, trunc, U, Vd, values, disc);
     truncation = disc;
+    rec.svd_seconds = lap(tp);
   }
   // SquareRescale(1.0) (MPS_matrix.hpp:66-70): Truncation/1, values to unit square sum
   double sq = 0.0;
@@ -179,7 +194,7 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
   } catch (const Error&) {
     guess.reset();   // inconsistent bond charges: next solve starts at random
   }
-  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  rec.block_seconds += lap(tp);
   rec.N = size;
   rec.energy = E;
   rec.energy_per_vertex = E / static_cast<double>(size);

@@ -11,6 +11,7 @@ struct IDMRGRecord {
   double energy, energy_per_vertex, entropy, truncation;
   int64_t chi, n;
   double matvecs_fp64, matvecs_fp32, residual, seconds, heff_flops;
+  double eig_seconds, svd_seconds, block_seconds;   // phases of `seconds`
 };
 
 TensorPtr make_dummy_left_block(Ctx& ctx, const BSTensor& W);
@@ -30,6 +31,7 @@ struct IDMRG {
   TensorPtr LBd, RBd, LB, RB, U, Vd, guess;
   std::vector<double> values, prev_lambda;
   double truncation = 0.0;
+  DevBuf<cplx> xbuf;   // eigenvector of the current step (grow-only)
 
   IDMRG(Ctx& c, TensorPtr mpo, const Charge& tgt, bool conv, bool mix, int k);
   IDMRGRecord grow(int64_t chi, double trunc);

@@ -3,7 +3,7 @@
 // algorithms 4.4 and 5.1), all on the device with rocBLAS/rocSOLVER:
 //   Y = A Omega (n x k Gaussian-like start), then q rounds of
 //   Q = orth(Y), Z = orth(A^H Q), Y = A Z;   Q = orth(Y);
-//   A^H Q = Q2 R  (QR),  R^H = Ur S Vr^H  (k x k SVD),
+//   A^H Q = Q2 R  (QR),  R^H = Ur S Vr^H  (k x k SVD, Jordan-Wielandt),
 //   U = Q Ur,  V^H = Vr^H Q2^H.
 // The singular values that matter are accepted only when their Ritz
 // residuals |A^H u_j - s_j v_j| are below `tol` (an absolute bound on the
@@ -97,8 +97,7 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   ncheck = std::min(ncheck, k);
   DevBuf<cplx> Y(m * k), W(n * k), R(k * k), Ur(k * k), Vrh(k * k), tau;
-  DevBuf<double> dS(k), dE(k), dres(std::max<int64_t>(ncheck, 1));
-  DevBuf<int> dinfo(1);
+  DevBuf<double> dS(k), dres(std::max<int64_t>(ncheck, 1));
   launch_random_fill(W.get(), n * k, 0x5eed0000ull + static_cast<uint64_t>(m * 31 + n), s);
   gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
   int q = 0;
@@ -125,11 +124,11 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
     st = rocsolver_zungqr(h, static_cast<rocblas_int>(n), static_cast<rocblas_int>(k), static_cast<rocblas_int>(k),
                           Z(W.get()), static_cast<rocblas_int>(n), Z(tau.get()));
     CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zungqr failed");
-    st = rocsolver_zgesvd(h, rocblas_svect_all, rocblas_svect_all, static_cast<rocblas_int>(k),
-                          static_cast<rocblas_int>(k), Z(R.get()), static_cast<rocblas_int>(k), dS.get(), Z(Ur.get()),
-                          static_cast<rocblas_int>(k), Z(Vrh.get()), static_cast<rocblas_int>(k), dE.get(),
-                          rocblas_outofplace, dinfo.get());
-    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgesvd failed");
+    // k x k core R^H = Ur S Vr^H through the Hermitian eigenproblem of
+    // [[0, R^H], [R, 0]] (divide and conquer; no squared condition)
+    std::vector<double> core;
+    jordan_wielandt_svd(ctx, R.get(), k, k, k, k, core, Ur.get(), Vrh.get());
+    dS.upload(core.data(), k, s);
     U.alloc(m * k);
     Vh.alloc(k * n);
     gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, k, Q.get(), m, Ur.get(), k, U.get(), m);
@@ -141,14 +140,10 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
     hipLaunchKernelGGL(ritz_residual_kernel, dim3(ncheck), dim3(256), 0, s, reinterpret_cast<double2*>(W.get()), n,
                        reinterpret_cast<const double2*>(Vh.get()), k, dS.get(), dres.get());
     HIP_CHECK(hipGetLastError());
-    S.resize(k);
+    S = core;
     std::vector<double> res(ncheck);
-    int info = 0;
-    HIP_CHECK(hipMemcpyAsync(S.data(), dS.get(), k * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(res.data(), dres.get(), ncheck * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    CAM_REQUIRE(info == 0, "truncated_svd: core SVD did not converge");
     double worst = 0.0;
     for (double r : res) worst = std::max(worst, std::sqrt(r));
     if (rounds) *rounds = q;

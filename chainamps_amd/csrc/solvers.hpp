@@ -22,6 +22,9 @@ double dense_lowest(Heff& H, cplx* x, double* info);
 double eigs_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv, bool mixed,
                    double* info);
 void gauge_fix(cplx* x, int64_t n, hipStream_t s);
+// entropy of singular values: sum over nonzero s of -s^2 log s^2
+// (sparse_interface.cpp:73-83)
+double entropy(const std::vector<double>& values);
 // contract with a diagonal on index k (values per position of that index)
 void scale_along_index(Ctx& ctx, BSTensor& T, int k, const std::vector<double>& lam);
 
@@ -31,6 +34,17 @@ void scale_along_index(Ctx& ctx, BSTensor& T, int k, const std::vector<double>& 
 // Vh is k x n, S descending.
 bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int64_t ncheck, double tol,
                    int max_rounds, std::vector<double>& S, DevBuf<cplx>& U, DevBuf<cplx>& Vh, int* rounds);
+
+// Hermitian eigendecomposition of the n x n device matrix A (column-major):
+// on return A holds the eigenvectors, w (device, n) the eigenvalues in
+// ascending order.  Divide and conquer, every eigenpair checked, QR
+// iteration as the fallback.
+void hermitian_eig(Ctx& ctx, cplx* A, int64_t n, double* w);
+// Leading k singular triplets of the m x n device matrix B (leading
+// dimension ldb) from the eigenpairs of [[0, B], [B^H, 0]]: S descending
+// (host), U m x k, Vh k x n (device, column-major).
+void jordan_wielandt_svd(Ctx& ctx, const cplx* B, int64_t ldb, int64_t m, int64_t n, int64_t k,
+                         std::vector<double>& S, cplx* U, cplx* Vh);
 
 // MPX_matrix::SVD with the reference truncation rule.
 void svd_truncate(Ctx& ctx, const BSTensor& theta, int64_t D, double max_trunc, TensorPtr& U,

@@ -129,9 +129,10 @@ int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, dou
  * algo 0 = four real MFMAs per complex product, 1 = three (Gauss);
  * tile_mode 0 = 64x64 tiles, 1 = per-block least padding, 2 = 32x64/64x32
  * only, 3 = least padding including 128x64 / 64x128 tiles; wg_per_cu =
- * workgroups per CU in the persistent grid.
- * get returns {algo, tile_mode, wg_per_cu} per stage (12 ints). */
-int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu);
+ * workgroups per CU in the persistent grid; kchunk = k per LDS stage (16,
+ * or 8 for half the LDS and more resident workgroups).
+ * get returns {algo, tile_mode, wg_per_cu, kchunk} per stage (16 ints). */
+int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu, int kchunk);
 int cam_heff_get_launch(const cam_heff* h, int precision, int* cfg);
 /* Reference order <-> device block layout (K3 gather/scatter). */
 int cam_heff_to_blocked(cam_heff* h, const double* in_dev, double* out_dev);
@@ -202,9 +203,10 @@ int cam_bond_evolution_operator(cam_ctx* ctx, const cam_tensor* H, double timest
  * MPO, target the target charges (nrules shorts).  converge=0 is the
  * reference growth-step eigensolver setting (tol 0.1); mixed=1 runs the
  * bulk of each eigensolve in FP32.  Each cam_idmrg_grow call adds two
- * vertices; record[13] receives {N, energy, energy/N, entropy,
+ * vertices; record[16] receives {N, energy, energy/N, entropy,
  * truncation, chi kept, n, matvecs fp64, matvecs fp32, residual, seconds,
- * heff flops, 0}. */
+ * heff flops, eigensolve seconds, SVD seconds, block update + H_eff build
+ * + prediction seconds, 0}. */
 typedef struct cam_idmrg_s cam_idmrg;
 int cam_idmrg_create(cam_ctx* ctx, const cam_tensor* W, const int16_t* target, int converge,
                      int mixed, int ncv, cam_idmrg** out);
@@ -234,6 +236,27 @@ int cam_itebd_lambda(const cam_itebd* it, int k, double* values, int64_t cap, in
 /* a copy of cell tensor k (0 or 1) as a new tensor handle */
 int cam_itebd_tensor(const cam_itebd* it, int k, cam_tensor** out);
 int cam_itebd_destroy(cam_itebd* it);
+
+/* ---- finite-chain TEBD ----------------------------------------------------
+ * TEBD (TEBD_routines.cpp:273-791) with the MPS in device memory instead of
+ * the "Evolving_<name>_Left_v" / "_Right_k" files: sites[0..nsites-1] is a
+ * left-canonical MPS (FiniteMPS::makeLC) of an even number of vertices,
+ * each [s (physical), a_l (in), a_r (out)]; W the lower-triangular model
+ * MPO; order 1, 2 or 4 (TrotterDecomposition, 14-85).  evolve runs nsteps
+ * time steps of apply_to_odd_bonds / apply_to_even_bonds / left_canonise
+ * with bond dimension D and truncation minS, measuring every `interval`
+ * steps (left_canonise_measure without MultiVertexMeasurements).  records
+ * returns 7 doubles per measurement: {step, time, largest truncation of the
+ * step, entropy of bond N/2, overlap with the initial state (re, im),
+ * |overlap|}, for up to cap records.  site returns a copy of left-canonical
+ * site v (1..N). */
+typedef struct cam_ftebd_s cam_ftebd;
+int cam_ftebd_create(cam_ctx* ctx, const cam_tensor* W, const cam_tensor* const* sites, int64_t nsites,
+                     double dt, int order, cam_ftebd** out);
+int cam_ftebd_evolve(cam_ftebd* t, int64_t nsteps, int64_t D, double minS, int64_t interval);
+int cam_ftebd_records(const cam_ftebd* t, double* records, int64_t cap, int64_t* n);
+int cam_ftebd_site(const cam_ftebd* t, int64_t v, cam_tensor** out);
+int cam_ftebd_destroy(cam_ftebd* t);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * Vector-Jacobian product of E = <psi|H_eff|psi>/<psi|psi>:

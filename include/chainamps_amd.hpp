@@ -425,7 +425,7 @@ class iDMRG {
     });
   }
   iDMRGStep grow(MPXInt chi, double truncation = 0.0) {
-    double r[13];
+    double r[16];
     check(cam_idmrg_grow(d_.get(), chi, truncation, r));
     iDMRGStep s;
     s.N = static_cast<MPXInt>(r[0]);

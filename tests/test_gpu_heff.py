@@ -78,18 +78,19 @@ def test_heff_matvec_parity(ctx, case):
 
 @pytest.mark.parametrize("algo,tile_mode,wg", [(0, 1, 4), (1, 1, 4), (1, 0, 2), (0, 2, 8), (1, 2, 1),
                                                (1, 3, 2), (0, 3, 4)])
-def test_heff_launch_configurations(ctx, algo, tile_mode, wg):
+@pytest.mark.parametrize("kc", [16, 8])
+def test_heff_launch_configurations(ctx, algo, tile_mode, wg, kc):
     """Every launch configuration the autotuner can pick (four or three real
     MFMAs per complex product, each tile policy, grid sizes) gives the same
     matvec to the same bar."""
-    print(f"case heff_launch algo={algo} tile_mode={tile_mode} wg={wg}", flush=True)
+    print(f"case heff_launch algo={algo} tile_mode={tile_mode} wg={wg} kc={kc}", flush=True)
     import chainamps_amd as cam
     rules, LB, W, RB = CASES["random_u1_w6"]()
     comp = dmrg.TwoVertexComponents(LB, W, RB)
     H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
     for prec in (cam.CAM_FP64, cam.CAM_FP32):
-        H.set_launch(prec, algo, tile_mode, wg)
-        assert H.get_launch(prec) == [(algo, tile_mode, wg)] * 4
+        H.set_launch(prec, algo, tile_mode, wg, kc)
+        assert H.get_launch(prec) == [(algo, tile_mode, wg, kc)] * 4
     rng = np.random.default_rng(12)
     v = rng.standard_normal(comp.n) + 1j * rng.standard_normal(comp.n)
     ref = comp.matvec(v)
