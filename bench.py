@@ -31,6 +31,38 @@ import numpy as np
 
 FP64_MFMA_PEAK = 78.6      # TFLOP/s, MI355X dense FP64 matrix
 FP32_MFMA_PEAK = 157.3     # TFLOP/s, MI355X dense FP32 matrix
+HERE = os.path.dirname(os.path.abspath(__file__))
+# HBM traffic of the dominant stage kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE and calibrated on a plain device copy; the summary
+# (with the commands that produced it) is committed next to the kernel trace
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "heff_matvec_chi500_d64.md")
+
+
+def committed_traffic(chi, d):
+    """Bytes per launch of stage L from the committed profile summary (the
+    line `traffic_bytes_per_launch: <value>`), or None for another workload
+    or when the summary is absent."""
+    if (chi, d) != (500, 64) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    import re
+    with open(TRAFFIC_PROFILE) as f:
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(gib=1.0, reps=3):
+    """A plain device-to-device copy of `gib` GiB, `reps` times: run under
+    rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), the counters per dispatch
+    against the known byte count give the calibration factors."""
+    import torch
+    n = int(gib * (1 << 30))
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "reps": reps, "read_bytes_per_copy": n, "write_bytes_per_copy": n}))
 
 
 def cpu_baseline(budget_s=20.0):
@@ -149,11 +181,16 @@ def main():
     ap.add_argument("--idmrg-steps", type=int, default=4,
                     help="growth steps per precision (the bond reaches chi at step 3 for d=64)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB device copy used to calibrate the HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
               "sharding of this path is a design, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
     import chainamps_amd as cam
@@ -193,6 +230,7 @@ def main():
     dom = int(np.argmax(ms64))
     achieved = fl[dom] / (ms64[dom] * 1e-3) / 1e12
     stage_names = ["L", "W1", "W2", "R"]
+    stage_bytes = H.stage_bytes()
     # eigensolve of one iDMRG growth step (the reference's setting for it:
     # converge=0, i.e. tol 0.1 and one Krylov cycle of ncv=20 vectors,
     # arpack_interface.cpp:28-39) from the same start vector, FP64 against
@@ -246,7 +284,10 @@ def main():
         "stages": {stage_names[i]: {"fp64_ms": round(ms64[i], 4), "fp32_ms": round(ms32[i], 4),
                                     "gflop": round(fl[i] / 1e9, 3)} for i in range(4)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4),
+                     "traffic": committed_traffic(a.chi, a.d) if stage_names[dom] == "L" else None,
+                     "traffic_source": "profiles/heff_matvec_chi500_d64.md (stage L, rocprofv3 --pmc)",
+                     "algorithmic_bytes": stage_bytes[dom],
                      "kernel": f"grouped_gemm_kernel<double> stage {stage_names[dom]}"},
         "cpu_baseline": cpu,
         "growth_step_eigensolve": eig or None,

@@ -74,6 +74,7 @@ _SIGS = {
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
     "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
+    "cam_heff_stage_bytes": [_p, _dp],
     "cam_heff_to_blocked": [_p, _p, _p],
     "cam_heff_from_blocked": [_p, _p, _p],
     "cam_heff_lowest": [_p, _p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _p, _dp],
@@ -312,6 +313,12 @@ class Heff:
         fl = np.zeros(4)
         _check(_LIB.cam_heff_profile(self.h, precision, reps, _ptr(ms, C.c_double), _ptr(fl, C.c_double)))
         return ms, fl
+
+    def stage_bytes(self):
+        """Algorithmic HBM bytes of each FP64 stage (L, W, W, R)."""
+        b = np.zeros(4)
+        _check(_LIB.cam_heff_stage_bytes(self.h, _ptr(b, C.c_double)))
+        return b.tolist()
 
     def set_launch(self, precision=CAM_FP64, algo=1, tile_mode=1, wg_per_cu=4, kchunk=16):
         """Fix the stage GEMM launch configuration (disables autotuning)."""

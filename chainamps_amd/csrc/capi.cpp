@@ -278,6 +278,17 @@ int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, dou
   CAM_API_END
 }
 
+int cam_heff_stage_bytes(const cam_heff* h, double* b) {
+  CAM_API_BEGIN
+  const cam::Heff& H = h->h;
+  const double e = sizeof(cam::cplx);
+  b[0] = e * double(H.n + H.L->total + H.t1n);
+  b[1] = e * double(H.t1n + H.W->total + H.t2n);
+  b[2] = e * double(H.t2n + H.W->total + H.t3n);
+  b[3] = e * double(H.t3n + H.R->total + H.n);
+  CAM_API_END
+}
+
 int cam_heff_set_launch(cam_heff* h, int precision, int algo, int tile_mode, int wg_per_cu, int kchunk) {
   CAM_API_BEGIN
   set_device(h->owner);

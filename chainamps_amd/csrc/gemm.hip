@@ -5,7 +5,7 @@
 // MFMA fragments.  K is consumed in chunks of KC (16 or 8) staged through LDS
 // with real and imaginary parts in separate planes (so one MFMA operand is one
 // LDS read); the next chunk is prefetched into registers while the current
-// one is multiplied.  A complex product is four real MFMAs,
+// one is multiplied (two chunks ahead).  A complex product is four real MFMAs,
 //   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre,
 // or three (Gauss): P1 = Are*Bre, P2 = Aim*Bim, P3 = (Are+Aim)*(Bre+Bim),
 //   Cre = P1 - P2,   Cim = P3 - P1 - P2.
@@ -59,6 +59,24 @@ __device__ __forceinline__ void gstore(T* p, const T& a) {
   *(__attribute__((address_space(1))) V*)(p) = v;
 }
 
+// The plan (tiles, problems, segments) is read-only for the whole launch:
+// reading it through the constant address space lets the compiler use
+// scalar loads, whose waits (lgkmcnt) do not also wait for the operand
+// prefetch in flight (vmcnt).
+template <typename T>
+__device__ __forceinline__ T cload(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "plan records are word-sized");
+  using W = const __attribute__((address_space(4))) uint32_t*;
+  const W q = (W)(p);
+  union {
+    T t;
+    uint32_t w[sizeof(T) / 4];
+  } u;
+#pragma unroll
+  for (int i = 0; i < static_cast<int>(sizeof(T) / 4); ++i) u.w[i] = q[i];
+  return u.t;
+}
+
 template <typename R> struct CT;
 template <> struct CT<double> { using T = double2; };
 template <> struct CT<float> { using T = float2; };
@@ -82,12 +100,10 @@ template <> struct Mf<float> {
 // G3: complex products with three real MFMAs (P1 = Ar Br, P2 = Ai Bi,
 // P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2) instead of four;
 // the error stays O(eps |A||B|) per product, as for the four-product form.
-// KC = 8 halves the LDS footprint and the prefetch registers: with the
-// register budget held to 168 (three waves per SIMD) three 64x64 FP64
-// workgroups fit on a CU instead of two (the 128-wide tiles keep their
-// registers: held to 168 they spill)
+// KC = 8 halves the LDS footprint and the prefetch registers (more resident
+// workgroups where registers allow) at one barrier per 8 instead of 16 k
 template <typename R, int TM, int TN, bool G3, int KC>
-__global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void grouped_gemm_kernel(const GemmProb* __restrict__ probs,
+__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __restrict__ probs,
                                                            const GemmSeg* __restrict__ segs,
                                                            const GemmTile* __restrict__ tiles,
                                                            int ntiles, GemmBases bases) {
@@ -115,8 +131,8 @@ __global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void gro
   const int G = gridDim.x;
   const int v0 = (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds : blockIdx.x;
   for (int t = v0; t < ntiles; t += G) {
-    const GemmTile tl = tiles[t];
-    const GemmProb P = probs[tl.prob];
+    const GemmTile tl = cload(tiles + t);
+    const GemmProb P = cload(probs + tl.prob);
     const int M = P.M, N = P.N;
     const int m0 = tl.m0, n0 = tl.n0;
 
@@ -132,20 +148,33 @@ __global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void gro
         if (G3) accs[G3 ? i : 0][G3 ? j : 0] = acc_t{0, 0, 0, 0};
       }
 
-    C ra[PA], rb[PB];
-    uint32_t amask = 0, bmask = 0;   // in-bounds flags of the staged elements
+    // two register sets of prefetched operands: chunk c+2 is loaded while
+    // chunk c is multiplied from LDS and chunk c+1 waits in the other set,
+    // so a global load has two chunks of MFMA work to arrive
+    C ra0[PA], rb0[PB], ra1[PA], rb1[PB];
+    uint32_t am0 = 0, bm0 = 0, am1 = 0, bm1 = 0;   // in-bounds flags of the staged elements
     const int send = P.seg0 + P.nseg;
-    int s = P.seg0;
-    while (s < send && segs[s].K <= 0) ++s;
-    int k0 = 0;
-    GemmSeg S;
-    if (s < send) S = segs[s];
+    // load cursor: segment and k offset of the next chunk to load (plans
+    // hold no empty segments)
+    int cs = P.seg0;
+    int ck = 0;
+    GemmSeg CS;
+    if (cs < send) CS = cload(segs + cs);
+    auto advance = [&]() {
+      ck += KC;
+      if (ck >= CS.K) {
+        ck = 0;
+        ++cs;
+        if (cs < send) CS = cload(segs + cs);
+      }
+      return cs < send;
+    };
 
     // operand loads: global address space (not flat, whose completion the
     // LDS waits of the MFMA loop would also have to wait for) and branch-free:
     // a padding element reads the block's first element and is zeroed when
     // it is staged to LDS, so nothing waits on the loads before the MFMAs
-    auto load = [&](const GemmSeg& G, int kk0) {
+    auto load = [&](const GemmSeg& G, int kk0, C(&ra)[PA], C(&rb)[PB], uint32_t& amask, uint32_t& bmask) {
       const int asp = (G.flags >> 8) & 15, bsp = (G.flags >> 12) & 15;
       const C* A = (asp ? reinterpret_cast<const C*>(bases.p[asp]) + reinterpret_cast<intptr_t>(G.A)
                         : reinterpret_cast<const C*>(G.A)) +
@@ -175,7 +204,8 @@ __global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void gro
         bmask = ok ? (bmask | (1u << p)) : (bmask & ~(1u << p));
       }
     };
-    auto store = [&](const GemmSeg& G, int buf) {
+    auto store = [&](const GemmSeg& G, int buf, const C(&ra)[PA], const C(&rb)[PB], uint32_t amask,
+                     uint32_t bmask) {
       R* __restrict__ pAr = sAr[buf];
       R* __restrict__ pAi = sAi[buf];
       R* __restrict__ pBr = sBr[buf];
@@ -218,27 +248,11 @@ __global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void gro
       }
     };
 
-    int buf = 0;
-    if (s < send) {
-      load(S, 0);
-      store(S, 0);
-    }
-    __syncthreads();
-    while (s < send) {
+    auto compute = [&](int buf) {
       const R* __restrict__ cAr = sAr[buf];
       const R* __restrict__ cAi = sAi[buf];
       const R* __restrict__ cBr = sBr[buf];
       const R* __restrict__ cBi = sBi[buf];
-      // advance (segment, k0) and prefetch the next chunk into registers
-      int ns = s, nk = k0 + KC;
-      GemmSeg NS = S;
-      if (nk >= S.K) {
-        nk = 0;
-        ++ns;
-        while (ns < send && segs[ns].K <= 0) ++ns;
-        if (ns < send) NS = segs[ns];
-      }
-      if (ns < send) load(NS, nk);
 #pragma unroll
       for (int ks = 0; ks < KC / 4; ++ks) {
         const int kr = ks * 4 + fk;
@@ -297,13 +311,45 @@ __global__ __launch_bounds__(256, (KC == 8 && TM * TN <= 4096 ? 3 : 1)) void gro
             for (int j = 0; j < FN; ++j) acci[i][j] = Mf<R>::mma(ai[i], br[j], acci[i][j]);
         }
       }
-      // the other buffer was last read before the previous barrier
-      if (ns < send) store(NS, buf ^ 1);
+    };
+
+    int buf = 0;
+    if (cs < send) {
+      GemmSeg S0 = CS;
+      load(S0, ck, ra0, rb0, am0, bm0);
+      store(S0, 0, ra0, rb0, am0, bm0);
+      bool v1 = advance();
+      GemmSeg S1 = CS;
+      load(S1, ck, ra1, rb1, am1, bm1);
       __syncthreads();
-      buf ^= 1;
-      s = ns;
-      k0 = nk;
-      S = NS;
+      // the loop body is written out for both register-set parities so that
+      // every set stays in registers.  Loads are issued unconditionally (past
+      // the last chunk they re-read the last segment and are never staged),
+      // so every path has the same loads in flight and the wait before a
+      // stage covers only the older set
+      for (;;) {
+        const bool v2 = v1 && advance();
+        const GemmSeg S2 = CS;
+        load(S2, ck, ra0, rb0, am0, bm0);
+        compute(buf);
+        // the other LDS buffer was last read before the previous barrier
+        if (v1) store(S1, buf ^ 1, ra1, rb1, am1, bm1);
+        __syncthreads();
+        buf ^= 1;
+        if (!v1) break;
+        const bool v3 = v2 && advance();
+        const GemmSeg S3 = CS;
+        load(S3, ck, ra1, rb1, am1, bm1);
+        compute(buf);
+        if (v2) store(S2, buf ^ 1, ra0, rb0, am0, bm0);
+        __syncthreads();
+        buf ^= 1;
+        if (!v2) break;
+        v1 = v3;
+        S1 = S3;
+      }
+    } else {
+      __syncthreads();
     }
 
     // epilogue
@@ -398,6 +444,7 @@ int GemmPlan::add_problem(void* C, int64_t csC, int M, int N, int batch, int64_t
 
 void GemmPlan::add_segment(int prob, const void* A, int64_t csA, int64_t bsA, const void* B,
                            int64_t ldB, int64_t bsB, int K, int flags, double are, double aim) {
+  if (K <= 0) return;   // contributes nothing; the kernels assume K >= 1
   GemmProb& p = probs[prob];
   CAM_REQUIRE(p.seg0 + p.nseg == static_cast<int32_t>(segs.size()),
               "segments of a problem must be added contiguously");

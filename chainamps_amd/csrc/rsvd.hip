@@ -5,9 +5,9 @@
 //   Q = orth(Y), Z = orth(A^H Q), Y = A Z;   Q = orth(Y);
 //   A^H Q = Q2 R  (QR),  R^H = Ur S Vr^H  (k x k SVD, Jordan-Wielandt),
 //   U = Q Ur,  V^H = Vr^H Q2^H.
-// The singular values that matter are accepted only when their Ritz
-// residuals |A^H u_j - s_j v_j| are below `tol` (an absolute bound on the
-// error of s_j); otherwise more rounds run, and the caller falls back to the
+// The singular values that matter are accepted only when their residuals
+// |A v_j - s_j u_j| are below `tol` (A^H u_j = s_j v_j holds by construction,
+// so this bounds the error of s_j); otherwise more rounds run, and the caller falls back to the
 // full decomposition when the budget is spent.  The work is GEMM-shaped
 // (k columns at a time), O(m n k) per round instead of O(m n min(m, n)).
 #include <rocblas/rocblas.h>
@@ -38,17 +38,18 @@ __global__ void upper_conj_transpose_kernel(const double2* __restrict__ qr, int6
   }
 }
 
-// E(:, j) -= conj(Vh(j, :))^T * s_j, then res[j] = |E(:, j)|^2 (one block per column)
-__global__ void ritz_residual_kernel(double2* __restrict__ E, int64_t n, const double2* __restrict__ Vh,
-                                     int64_t k, const double* __restrict__ s, double* __restrict__ res) {
+// E(:, j) -= s_j U(:, j), then res[j] = |E(:, j)|^2 (one block per column;
+// E and U are m x ncheck / m x k, column-major)
+__global__ void ritz_residual_kernel(double2* __restrict__ E, int64_t m, const double2* __restrict__ U,
+                                     const double* __restrict__ s, double* __restrict__ res) {
   __shared__ double red[256];
   const int64_t j = blockIdx.x;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const double2 v = Vh[j + i * k];
-    double2 e = E[i + j * n];
-    e.x -= v.x * s[j];
-    e.y += v.y * s[j];
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+    const double2 u = U[i + j * m];
+    double2 e = E[i + j * m];
+    e.x -= u.x * s[j];
+    e.y -= u.y * s[j];
     acc += e.x * e.x + e.y * e.y;
   }
   red[threadIdx.x] = acc;
@@ -96,7 +97,7 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   hipStream_t s = ctx.stream;
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   ncheck = std::min(ncheck, k);
-  DevBuf<cplx> Y(m * k), W(n * k), R(k * k), Ur(k * k), Vrh(k * k), tau;
+  DevBuf<cplx> Y(m * k), W(n * k), R(k * k), Ur(k * k), Vrh(k * k), E(m * std::max<int64_t>(ncheck, 1)), tau;
   DevBuf<double> dS(k), dres(std::max<int64_t>(ncheck, 1));
   launch_random_fill(W.get(), n * k, 0x5eed0000ull + static_cast<uint64_t>(m * 31 + n), s);
   gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
@@ -134,11 +135,13 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
     gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, k, Q.get(), m, Ur.get(), k, U.get(), m);
     gemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose, k, n, k, Vrh.get(), k, W.get(), n,
          Vh.get(), k);
-    // Ritz residuals |A^H u_j - s_j v_j| of the leading ncheck triplets
-    gemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, ncheck, m, A, m, U.get(), m, W.get(),
-         n);
-    hipLaunchKernelGGL(ritz_residual_kernel, dim3(ncheck), dim3(256), 0, s, reinterpret_cast<double2*>(W.get()), n,
-                       reinterpret_cast<const double2*>(Vh.get()), k, dS.get(), dres.get());
+    // residuals |A v_j - s_j u_j| of the leading ncheck triplets: by
+    // construction A^H u_j = s_j v_j exactly, so |s_j - sigma| <= this
+    // residual for some singular value sigma of A
+    gemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose, m, ncheck, n, A, m, Vh.get(), k, E.get(),
+         m);
+    hipLaunchKernelGGL(ritz_residual_kernel, dim3(ncheck), dim3(256), 0, s, reinterpret_cast<double2*>(E.get()), m,
+                       reinterpret_cast<const double2*>(U.get()), dS.get(), dres.get());
     HIP_CHECK(hipGetLastError());
     S = core;
     std::vector<double> res(ncheck);

@@ -37,9 +37,11 @@ namespace {
 // Blocks with min(rows, cols) >= this and room for at most half their values
 // use the leading-values decomposition (CAM_SVD_EXACT=1 disables it).
 constexpr int64_t kTruncatedMinDim = 1536;
-// Ritz residual bound of accepted values, relative to the block's
-// Frobenius norm (an absolute bound on each value's error).
-constexpr double kRitzTol = 2e-15;
+// Residual bound |A v - s u| of accepted triplets, relative to the block's
+// Frobenius norm: each accepted value is within this of a singular value
+// of the block (the FP64 floor of the residual itself is a few 1e-15 for
+// blocks of 10^4 rows).
+constexpr double kRitzTol = 5e-14;
 
 struct FlagJob {
   const double2* in;

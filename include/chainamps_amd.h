@@ -122,6 +122,10 @@ int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int preci
  * (L, W, W, R), the average launch duration in ms measured with HIP events
  * on the context stream and the algorithmic flops of that stage. */
 int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, double* stage_flops);
+/* Algorithmic HBM bytes per stage (L, W, W, R) of the FP64 matvec: every
+ * operand read once and the output written once (psi -> T1 -> T2 -> T3 ->
+ * out, plus the environment or MPO blocks), 16 bytes per element. */
+int cam_heff_stage_bytes(const cam_heff* h, double* stage_bytes);
 /* Launch configuration of the four stage GEMMs (no reference counterpart:
  * the reference's SpGEMM has no tunables).  By default the first matvec at
  * each precision autotunes them (timed candidates, cached per block-size

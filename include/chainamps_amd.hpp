@@ -409,6 +409,60 @@ class iTEBD {
   std::shared_ptr<cam_itebd> it_;
 };
 
+/* Finite-chain TEBD (TEBD::evolve, TEBD_routines.cpp:672-791) on an MPS held
+ * in device memory: `sites` is a left-canonical MPS of an even number of
+ * vertices, each MPS_matrix [s, a_l, a_r]; Trotter order 1, 2 or 4. */
+struct TEBDMeasurement {
+  MPXInt TimeStep = 0;
+  double Time = 0.0, Truncation = 0.0, Entropy = 0.0;
+  std::complex<double> Overlap;
+};
+class TEBD {
+ public:
+  TEBD(const MPX_matrix& H, const std::vector<MPX_matrix>& sites, double time_step_size, int order = 2) {
+    std::vector<const cam_tensor*> h;
+    for (const MPX_matrix& m : sites) h.push_back(m.handle());
+    cam_ftebd* t = nullptr;
+    check(cam_ftebd_create(Context::get(), H.handle(), h.data(), static_cast<int64_t>(h.size()), time_step_size,
+                           order, &t));
+    t_ = std::shared_ptr<cam_ftebd>(t, [](cam_ftebd* p) {
+      if (p) cam_ftebd_destroy(p);
+    });
+    n_ = static_cast<MPXInt>(sites.size());
+  }
+  void evolve(MPXInt num_steps, MPXInt bond_dimension, double minS, MPXInt measurement_interval = 1) {
+    check(cam_ftebd_evolve(t_.get(), num_steps, bond_dimension, minS, measurement_interval));
+  }
+  /* one entry per measurement (left_canonise_measure) so far */
+  std::vector<TEBDMeasurement> results() const {
+    int64_t n = 0;
+    check(cam_ftebd_records(t_.get(), nullptr, 0, &n));
+    std::vector<double> r(static_cast<size_t>(7 * std::max<int64_t>(n, 1)));
+    check(cam_ftebd_records(t_.get(), r.data(), n, &n));
+    std::vector<TEBDMeasurement> out(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+      const double* x = r.data() + 7 * i;
+      out[i].TimeStep = static_cast<MPXInt>(x[0]);
+      out[i].Time = x[1];
+      out[i].Truncation = x[2];
+      out[i].Entropy = x[3];
+      out[i].Overlap = std::complex<double>(x[4], x[5]);
+    }
+    return out;
+  }
+  /* left-canonical site v = 1..N after the last step */
+  MPX_matrix Matrix(MPXInt v) const {
+    cam_tensor* t = nullptr;
+    check(cam_ftebd_site(t_.get(), v, &t));
+    return MPX_matrix(t);
+  }
+  MPXInt size() const { return n_; }
+
+ private:
+  std::shared_ptr<cam_ftebd> t_;
+  MPXInt n_ = 0;
+};
+
 /* iDMRG growth (iDMRG::run / SuperBlock::grow_two_vertex). */
 struct iDMRGStep {
   MPXInt N = 0, chi = 0, n = 0;
