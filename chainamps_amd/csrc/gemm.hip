@@ -5,7 +5,7 @@
 // MFMA fragments.  K is consumed in chunks of KC (16 or 8) staged through LDS
 // with real and imaginary parts in separate planes (so one MFMA operand is one
 // LDS read); the next chunk is prefetched into registers while the current
-// one is multiplied (two chunks ahead).  A complex product is four real MFMAs,
+// one is multiplied.  A complex product is four real MFMAs,
 //   Cre += Are*Bre + (-Aim)*Bim,   Cim += Are*Bim + Aim*Bre,
 // or three (Gauss): P1 = Are*Bre, P2 = Aim*Bim, P3 = (Are+Aim)*(Bre+Bim),
 //   Cre = P1 - P2,   Cim = P3 - P1 - P2.
@@ -148,11 +148,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
         if (G3) accs[G3 ? i : 0][G3 ? j : 0] = acc_t{0, 0, 0, 0};
       }
 
-    // two register sets of prefetched operands: chunk c+2 is loaded while
-    // chunk c is multiplied from LDS and chunk c+1 waits in the other set,
-    // so a global load has two chunks of MFMA work to arrive
-    C ra0[PA], rb0[PB], ra1[PA], rb1[PB];
-    uint32_t am0 = 0, bm0 = 0, am1 = 0, bm1 = 0;   // in-bounds flags of the staged elements
+    // prefetched operands of the next chunk (registers) and their in-bounds flags
+    C ra0[PA], rb0[PB];
+    uint32_t am0 = 0, bm0 = 0;
     const int send = P.seg0 + P.nseg;
     // load cursor: segment and k offset of the next chunk to load (plans
     // hold no empty segments)
@@ -253,19 +251,41 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       const R* __restrict__ cAi = sAi[buf];
       const R* __restrict__ cBr = sBr[buf];
       const R* __restrict__ cBi = sBi[buf];
-#pragma unroll
-      for (int ks = 0; ks < KC / 4; ++ks) {
+      // fragments of step ks+1 are read from LDS before the MFMAs of step ks
+      // are issued, so their latency hides behind those MFMAs (tiles up to
+      // 64x64; the 128-wide tiles have no registers to spare and read each
+      // step's fragments just before its MFMAs)
+      constexpr bool kPipe = TM * TN <= 4096;
+      R far_[2][FM], fai_[2][FM], fbr_[2][FN], fbi_[2][FN];
+      auto frag = [&](int ks, int slot) {
         const int kr = ks * 4 + fk;
-        R ar[FM], ai[FM], br[FN], bi[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          ar[i] = cAr[kr * LDA + wm * (TM / 2) + i * 16 + fr];
-          ai[i] = cAi[kr * LDA + wm * (TM / 2) + i * 16 + fr];
+          far_[slot][i] = cAr[kr * LDA + wm * (TM / 2) + i * 16 + fr];
+          fai_[slot][i] = cAi[kr * LDA + wm * (TM / 2) + i * 16 + fr];
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          br[j] = cBr[kr * LDB + wn * (TN / 2) + j * 16 + fr];
-          bi[j] = cBi[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+          fbr_[slot][j] = cBr[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+          fbi_[slot][j] = cBi[kr * LDB + wn * (TN / 2) + j * 16 + fr];
+        }
+      };
+      if (kPipe) frag(0, 0);
+#pragma unroll
+      for (int ks = 0; ks < KC / 4; ++ks) {
+        const int cur = kPipe ? (ks & 1) : 0;
+        if (!kPipe) frag(ks, 0);
+        else if (ks + 1 < KC / 4) frag(ks + 1, cur ^ 1);
+        R ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          ar[i] = far_[cur][i];
+          ai[i] = fai_[cur][i];
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          br[j] = fbr_[cur][j];
+          bi[j] = fbi_[cur][j];
         }
         if constexpr (G3) {
           // three passes; an accumulator is revisited FM*FN MFMAs later
@@ -318,35 +338,25 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       GemmSeg S0 = CS;
       load(S0, ck, ra0, rb0, am0, bm0);
       store(S0, 0, ra0, rb0, am0, bm0);
-      bool v1 = advance();
-      GemmSeg S1 = CS;
-      load(S1, ck, ra1, rb1, am1, bm1);
       __syncthreads();
-      // the loop body is written out for both register-set parities so that
-      // every set stays in registers.  Loads are issued unconditionally (past
-      // the last chunk they re-read the last segment and are never staged),
-      // so every path has the same loads in flight and the wait before a
-      // stage covers only the older set
+      // chunk c+1 is loaded into registers while chunk c is multiplied from
+      // LDS, then staged into the other LDS buffer.  The load is issued
+      // unconditionally (past the last chunk it re-reads the last segment and
+      // is never staged), so every path has the same loads in flight.
       for (;;) {
-        const bool v2 = v1 && advance();
-        const GemmSeg S2 = CS;
-        load(S2, ck, ra0, rb0, am0, bm0);
+        const bool v1 = advance();
+        const GemmSeg S1 = CS;
+        load(S1, ck, ra0, rb0, am0, bm0);
+        // keep every load of the next chunk ahead of the MFMAs (the scheduler
+        // would otherwise sink some of them next to their LDS stores, where
+        // their whole latency is exposed)
+        __builtin_amdgcn_sched_barrier(0);
         compute(buf);
         // the other LDS buffer was last read before the previous barrier
-        if (v1) store(S1, buf ^ 1, ra1, rb1, am1, bm1);
+        if (v1) store(S1, buf ^ 1, ra0, rb0, am0, bm0);
         __syncthreads();
         buf ^= 1;
         if (!v1) break;
-        const bool v3 = v2 && advance();
-        const GemmSeg S3 = CS;
-        load(S3, ck, ra1, rb1, am1, bm1);
-        compute(buf);
-        if (v2) store(S2, buf ^ 1, ra0, rb0, am0, bm0);
-        __syncthreads();
-        buf ^= 1;
-        if (!v2) break;
-        v1 = v3;
-        S1 = S3;
       }
     } else {
       __syncthreads();
@@ -505,11 +515,35 @@ void GemmPlan::build_tiles_mode(int mode) {
           cost[shape].push_back(double(ktot) + 8.0);
         }
   }
-  // longest first (LPT) for the persistent round-robin
+  // Longest first (LPT) for the persistent round-robin, by cost class
+  // (within 25 %), and inside a class the tiles that read the same B
+  // columns (problems whose first segment has the same B operand, e.g. the
+  // three MPO channels applied to one psi block) next to each other, M-tiles
+  // innermost: consecutive tiles run at the same time on one XCD, so B is
+  // fetched from HBM once and re-read from that XCD's L2.
+  auto bkey = [&](const GemmTile& g) -> uint64_t {
+    const GemmProb& p = probs[g.prob];
+    if (!p.nseg) return 0;
+    const GemmSeg& q = segs[p.seg0];
+    return (static_cast<uint64_t>((q.flags >> 12) & 15) << 56) ^ reinterpret_cast<uint64_t>(q.B) ^
+           (static_cast<uint64_t>(g.batch) << 40);
+  };
   for (int t = 0; t < kNumTileShapes; ++t) {
     std::vector<size_t> ord(T[t].size());
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[t][a] > cost[t][b]; });
+    std::vector<int> cls(T[t].size());
+    std::vector<uint64_t> bk(T[t].size());
+    for (size_t i = 0; i < ord.size(); ++i) {
+      cls[i] = static_cast<int>(std::floor(std::log(cost[t][i]) / std::log(1.25)));
+      bk[i] = bkey(T[t][i]);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+      if (cls[a] != cls[b]) return cls[a] > cls[b];
+      if (bk[a] != bk[b]) return bk[a] < bk[b];
+      if (T[t][a].n0 != T[t][b].n0) return T[t][a].n0 < T[t][b].n0;
+      if (T[t][a].prob != T[t][b].prob) return T[t][a].prob < T[t][b].prob;
+      return T[t][a].m0 < T[t][b].m0;
+    });
     std::vector<GemmTile> sorted;
     sorted.reserve(ord.size());
     for (auto o : ord) sorted.push_back(T[t][o]);

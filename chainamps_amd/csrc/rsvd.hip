@@ -3,7 +3,7 @@
 // algorithms 4.4 and 5.1), all on the device with rocBLAS/rocSOLVER:
 //   Y = A Omega (n x k Gaussian-like start), then q rounds of
 //   Q = orth(Y), Z = orth(A^H Q), Y = A Z;   Q = orth(Y);
-//   A^H Q = Q2 R  (QR),  R^H = Ur S Vr^H  (k x k SVD, Jordan-Wielandt),
+//   A^H Q = Q2 R  (Cholesky QR),  R^H = Ur S Vr^H  (k x k SVD, Jordan-Wielandt),
 //   U = Q Ur,  V^H = Vr^H Q2^H.
 // The singular values that matter are accepted only when their residuals
 // |A v_j - s_j u_j| are below `tol` (A^H u_j = s_j v_j holds by construction,
@@ -15,28 +15,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 #include "solvers.hpp"
 
 namespace cam {
 
 namespace {
-
-// out (k x k) = R^H with R the upper triangle of the k x k top of `qr` (ld)
-__global__ void upper_conj_transpose_kernel(const double2* __restrict__ qr, int64_t ld, int64_t k,
-                                            double2* __restrict__ out) {
-  const int64_t tot = k * k;
-  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot;
-       e += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t i = e % k, j = e / k;   // out(i, j) = conj(R(j, i)), R(j, i) = 0 for j > i
-    double2 v = make_double2(0.0, 0.0);
-    if (j <= i) {
-      const double2 r = qr[j + i * ld];
-      v = make_double2(r.x, -r.y);
-    }
-    out[e] = v;
-  }
-}
 
 // E(:, j) -= s_j U(:, j), then res[j] = |E(:, j)|^2 (one block per column;
 // E and U are m x ncheck / m x k, column-major)
@@ -78,7 +63,7 @@ void gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int64_t 
 }
 
 // X (m x k, ld m) <- orthonormal basis of its columns (Householder QR, explicit Q)
-void orthonormalise(rocblas_handle h, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& tau) {
+void householder_orthonormalise(rocblas_handle h, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& tau) {
   tau.ensure(std::max<int64_t>(k, 1));
   rocblas_status st = rocsolver_zgeqrf(h, static_cast<rocblas_int>(m), static_cast<rocblas_int>(k), Z(X),
                                        static_cast<rocblas_int>(m), Z(tau.get()));
@@ -86,6 +71,67 @@ void orthonormalise(rocblas_handle h, cplx* X, int64_t m, int64_t k, DevBuf<cplx
   st = rocsolver_zungqr(h, static_cast<rocblas_int>(m), static_cast<rocblas_int>(k), static_cast<rocblas_int>(k), Z(X),
                         static_cast<rocblas_int>(m), Z(tau.get()));
   CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zungqr failed");
+}
+
+// G (k x k, upper triangle) += c * trace(G) on the diagonal (one workgroup)
+__global__ void shift_diagonal_kernel(double2* __restrict__ G, int64_t k, double c) {
+  __shared__ double red[256];
+  double t = 0.0;
+  for (int64_t i = threadIdx.x; i < k; i += blockDim.x) t += G[i + i * k].x;
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double sh = c * red[0];
+  for (int64_t i = threadIdx.x; i < k; i += blockDim.x) G[i + i * k].x += sh;
+}
+
+// One Cholesky QR pass: G = X^H X (+ shift), R = chol(G), X <- X R^{-1}.
+// False when the factorization breaks down.
+bool cholqr_pass(Ctx& ctx, cplx* X, int64_t m, int64_t k, cplx* G, int* dinfo, double shift) {
+  rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
+  hipStream_t s = ctx.stream;
+  const double one = 1.0, zero = 0.0;
+  rocblas_status st = rocblas_zherk(h, rocblas_fill_upper, rocblas_operation_conjugate_transpose,
+                                    static_cast<rocblas_int>(k), static_cast<rocblas_int>(m), &one, Z(X),
+                                    static_cast<rocblas_int>(m), &zero, Z(G), static_cast<rocblas_int>(k));
+  CAM_REQUIRE(st == rocblas_status_success, "rocblas_zherk failed");
+  if (shift > 0.0) {
+    hipLaunchKernelGGL(shift_diagonal_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<double2*>(G), k, shift);
+    HIP_CHECK(hipGetLastError());
+  }
+  st = rocsolver_zpotrf(h, rocblas_fill_upper, static_cast<rocblas_int>(k), Z(G), static_cast<rocblas_int>(k), dinfo);
+  CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zpotrf failed");
+  int info = 0;
+  HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (info != 0) return false;
+  st = rocblas_ztrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                     static_cast<rocblas_int>(m), static_cast<rocblas_int>(k), &kOne, Z(G),
+                     static_cast<rocblas_int>(k), Z(X), static_cast<rocblas_int>(m));
+  CAM_REQUIRE(st == rocblas_status_success, "rocblas_ztrsm failed");
+  return true;
+}
+
+// X (m x k) <- orthonormal basis of its columns by shifted Cholesky QR3
+// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
+// 42 (2020) A477): one pass with the shift 11 (m k + k (k + 1)) u |X|_F^2,
+// which keeps the Cholesky factorization defined for any condition below
+// 1/u, then two plain passes; three GEMM-shaped passes instead of a
+// Householder QR.  Householder QR takes over if a factorization breaks down
+// (numerically rank-deficient X).
+void orthonormalise(Ctx& ctx, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& G, DevBuf<int>& dinfo,
+                    DevBuf<cplx>& tau) {
+  G.ensure(static_cast<size_t>(k) * k);
+  dinfo.ensure(1);
+  const double u = 0.5 * std::numeric_limits<double>::epsilon();
+  const double c = 11.0 * (double(m) * double(k) + double(k) * double(k + 1)) * u;
+  if (cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), c) && cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), 0.0) &&
+      cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), 0.0))
+    return;
+  householder_orthonormalise(static_cast<rocblas_handle>(ctx.blas_handle()), X, m, k, tau);
 }
 
 }  // namespace
@@ -97,7 +143,9 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   hipStream_t s = ctx.stream;
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   ncheck = std::min(ncheck, k);
-  DevBuf<cplx> Y(m * k), W(n * k), R(k * k), Ur(k * k), Vrh(k * k), E(m * std::max<int64_t>(ncheck, 1)), tau;
+  DevBuf<cplx> Y(m * k), W(n * k), Wc(n * k), R(k * k), Ur(k * k), Vrh(k * k), E(m * std::max<int64_t>(ncheck, 1)),
+      tau, G;
+  DevBuf<int> dinfo;
   DevBuf<double> dS(k), dres(std::max<int64_t>(ncheck, 1));
   launch_random_fill(W.get(), n * k, 0x5eed0000ull + static_cast<uint64_t>(m * 31 + n), s);
   gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
@@ -105,26 +153,21 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   for (;;) {
     // two subspace rounds between residual checks
     for (int r = 0; r < 2; ++r, ++q) {
-      orthonormalise(h, Y.get(), m, k, tau);
+      orthonormalise(ctx, Y.get(), m, k, G, dinfo, tau);
       gemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, k, m, A, m, Y.get(), m, W.get(), n);
-      orthonormalise(h, W.get(), n, k, tau);
+      orthonormalise(ctx, W.get(), n, k, G, dinfo, tau);
       gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
     }
     // Rayleigh-Ritz on span(Y)
     DevBuf<cplx> Q(m * k);
     HIP_CHECK(hipMemcpyAsync(Q.get(), Y.get(), m * k * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-    orthonormalise(h, Q.get(), m, k, tau);
+    orthonormalise(ctx, Q.get(), m, k, G, dinfo, tau);
+    // A^H Q = Q2 R: Q2 by Cholesky QR of a copy, R^H = (A^H Q)^H Q2
     gemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, k, m, A, m, Q.get(), m, W.get(), n);
-    tau.ensure(k);
-    rocblas_status st = rocsolver_zgeqrf(h, static_cast<rocblas_int>(n), static_cast<rocblas_int>(k), Z(W.get()),
-                                         static_cast<rocblas_int>(n), Z(tau.get()));
-    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgeqrf failed");
-    hipLaunchKernelGGL(upper_conj_transpose_kernel, dim3(grid_of(k * k)), dim3(256), 0, s,
-                       reinterpret_cast<const double2*>(W.get()), n, k, reinterpret_cast<double2*>(R.get()));
-    HIP_CHECK(hipGetLastError());
-    st = rocsolver_zungqr(h, static_cast<rocblas_int>(n), static_cast<rocblas_int>(k), static_cast<rocblas_int>(k),
-                          Z(W.get()), static_cast<rocblas_int>(n), Z(tau.get()));
-    CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zungqr failed");
+    HIP_CHECK(hipMemcpyAsync(Wc.get(), W.get(), n * k * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+    orthonormalise(ctx, W.get(), n, k, G, dinfo, tau);
+    gemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, k, k, n, Wc.get(), n, W.get(), n, R.get(),
+         k);
     // k x k core R^H = Ur S Vr^H through the Hermitian eigenproblem of
     // [[0, R^H], [R, 0]] (divide and conquer; no squared condition)
     std::vector<double> core;

@@ -22,7 +22,9 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 #include <map>
 #include <cstdlib>
@@ -36,7 +38,7 @@ namespace {
 
 // Blocks with min(rows, cols) >= this and room for at most half their values
 // use the leading-values decomposition (CAM_SVD_EXACT=1 disables it).
-constexpr int64_t kTruncatedMinDim = 1536;
+constexpr int64_t kTruncatedMinDim = 1024;
 // Residual bound |A v - s u| of accepted triplets, relative to the block's
 // Frobenius norm: each accepted value is within this of a singular value
 // of the block (the FP64 floor of the residual itself is a few 1e-15 for
@@ -234,6 +236,9 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
   DevBuf<cplx> partial(64 * 1024), dcoef(64);
   const char* ev = std::getenv("CAM_SVD_EXACT");
   const bool exact_only = ev && *ev && *ev != '0';
+  // CAM_SVD_TRACE=1 prints the path and time of every decomposed block
+  const char* tv = std::getenv("CAM_SVD_TRACE");
+  const bool trace = tv && *tv && *tv != '0';
   // the full (economy) decomposition of one group, as the reference's
   // per-block zgesdd
   auto full_svd = [&](Group& g) {
@@ -319,8 +324,15 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
     if (!exact_only && D > 0 && kfull >= kTruncatedMinDim && 2 * kk <= kfull) {
       std::vector<double> S;
       int rounds = 0;
-      if (truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, std::min(D, kk), kRitzTol * std::sqrt(g.weight), 24, S,
-                        g.U, g.Vh, &rounds)) {
+      const auto t0 = std::chrono::steady_clock::now();
+      const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, std::min(D, kk), kRitzTol * std::sqrt(g.weight),
+                                    24, S, g.U, g.Vh, &rounds);
+      if (trace)
+        std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld: %s after %d rounds, %.3f s\n",
+                     static_cast<long long>(g.mg), static_cast<long long>(g.ng), static_cast<long long>(kk),
+                     ok ? "accepted" : "not accepted", rounds,
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      if (ok) {
         g.S = std::move(S);
         g.k = kk;
         g.truncated = true;
@@ -330,7 +342,14 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
         continue;   // g.A is kept in case the full decomposition is needed
       }
     }
+    const auto t1 = std::chrono::steady_clock::now();
     full_svd(g);
+    if (trace) {
+      HIP_CHECK(hipStreamSynchronize(s));
+      std::fprintf(stderr, "svd_truncate: block %lld x %lld, full decomposition, %.3f s\n",
+                   static_cast<long long>(g.mg), static_cast<long long>(g.ng),
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
+    }
   }
 
   // ---- global sort and truncation rule --------------------------------------
