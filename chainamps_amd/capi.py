@@ -243,6 +243,21 @@ class Tensor:
         _check(_LIB.cam_tensor_rescale(self.h, f.real, f.imag))
         return self
 
+    def store(self, path):
+        """MPX_matrix::store: the reference's .MPX_matrix / .MPS_matrix format."""
+        _check(_LIB.cam_tensor_store(self.h, os.fsencode(path)))
+
+    @staticmethod
+    def load(ctx, path, rules, phys_charges):
+        """load_MPX_matrix: physical indices take the basis charges
+        phys_charges (d x nrules)."""
+        rules = np.ascontiguousarray(rules, dtype=np.int16)
+        ch = np.ascontiguousarray(np.asarray(phys_charges, dtype=np.int16).reshape(-1, len(rules)))
+        h = _p()
+        _check(_LIB.cam_tensor_load(ctx.h, os.fsencode(path), len(rules), _ptr(rules, C.c_int16), ch.shape[0],
+                                    _ptr(ch, C.c_int16), C.byref(h)))
+        return Tensor(ctx, h)
+
     def close(self):
         if getattr(self, "h", None):
             _LIB.cam_tensor_destroy(self.h)
@@ -445,7 +460,7 @@ _SIGS.update(_SIGS_IDMRG)
 
 IDMRG_FIELDS = ("N", "energy", "energy_per_vertex", "entropy", "truncation", "chi", "n",
                 "matvecs_fp64", "matvecs_fp32", "residual", "seconds", "heff_flops",
-                "eig_seconds", "svd_seconds", "block_seconds")
+                "eig_seconds", "svd_seconds", "block_seconds", "fidelity")
 
 
 class IDMRG:
@@ -598,3 +613,65 @@ class FTEBD:
             self.close()
         except Exception:
             pass
+
+
+_SIGS_STORAGE = {
+    "cam_tensor_store": [_p, C.c_char_p],
+    "cam_tensor_load": [_p, C.c_char_p, C.c_int, C.POINTER(C.c_int16), C.c_int64, C.POINTER(C.c_int16),
+                        C.POINTER(_p)],
+    "cam_unitcell_store": [C.c_char_p, C.c_int, C.POINTER(C.c_int16), C.c_int64, C.POINTER(C.c_int16), _dp, C.c_int,
+                           C.POINTER(_p), C.POINTER(_dp), _i64p],
+    "cam_unitcell_load": [_p, C.c_char_p, C.POINTER(_p)],
+    "cam_unitcell_basis": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int16), _i64p, C.POINTER(C.c_int16), _dp,
+                           C.c_int64],
+    "cam_unitcell_size": [_p, C.POINTER(C.c_int)],
+    "cam_unitcell_matrix": [_p, C.c_int, C.POINTER(_p)],
+    "cam_unitcell_lambda": [_p, C.c_int, _dp, C.c_int64, _i64p],
+    "cam_unitcell_destroy": [_p],
+}
+_SIGS.update(_SIGS_STORAGE)
+
+
+def store_unitcell(path, rules, phys_charges, energies, matrices, lambdas):
+    """UnitCell::store: basis (rules, charges d x nrules, energies), the MPS
+    matrices and their lambdas, in the reference's .UNITCELL format."""
+    rules = np.ascontiguousarray(rules, dtype=np.int16)
+    ch = np.ascontiguousarray(np.asarray(phys_charges, dtype=np.int16).reshape(-1, len(rules)))
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    lam = [np.ascontiguousarray(l, dtype=np.float64) for l in lambdas]
+    n = len(matrices)
+    mh = (_p * max(n, 1))(*[m.h for m in matrices])
+    lp = (_dp * max(n, 1))(*[_ptr(l, C.c_double) for l in lam])
+    ls = np.array([len(l) for l in lam] or [0], dtype=np.int64)
+    _check(_LIB.cam_unitcell_store(os.fsencode(path), len(rules), _ptr(rules, C.c_int16), ch.shape[0],
+                                   _ptr(ch, C.c_int16), _ptr(en, C.c_double), n, mh, lp, _ptr(ls, C.c_int64)))
+
+
+def load_unitcell(ctx, path):
+    """load_UnitCell_binary: returns (rules, charges, energies, matrices,
+    lambdas)."""
+    h = _p()
+    _check(_LIB.cam_unitcell_load(ctx.h, os.fsencode(path), C.byref(h)))
+    try:
+        nr, d = C.c_int(), C.c_int64()
+        _check(_LIB.cam_unitcell_basis(h, C.byref(nr), None, C.byref(d), None, None, 0))
+        rules = np.zeros(nr.value, np.int16)
+        ch = np.zeros((d.value, nr.value), np.int16)
+        en = np.zeros(d.value)
+        _check(_LIB.cam_unitcell_basis(h, None, _ptr(rules, C.c_int16), None, _ptr(ch, C.c_int16),
+                                       _ptr(en, C.c_double), d.value))
+        n = C.c_int()
+        _check(_LIB.cam_unitcell_size(h, C.byref(n)))
+        mats, lams = [], []
+        for k in range(n.value):
+            t = _p()
+            _check(_LIB.cam_unitcell_matrix(h, k, C.byref(t)))
+            mats.append(Tensor(ctx, t))
+            m = C.c_int64()
+            _check(_LIB.cam_unitcell_lambda(h, k, None, 0, C.byref(m)))
+            v = np.zeros(max(m.value, 1))
+            _check(_LIB.cam_unitcell_lambda(h, k, _ptr(v, C.c_double), len(v), C.byref(m)))
+            lams.append(v[:m.value])
+        return rules, ch, en, mats, lams
+    finally:
+        _LIB.cam_unitcell_destroy(h)

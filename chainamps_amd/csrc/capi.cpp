@@ -10,6 +10,7 @@
 #include "idmrg.hpp"
 #include "itebd.hpp"
 #include "ftebd.hpp"
+#include "storage.hpp"
 #include <memory>
 
 using namespace cam;
@@ -440,6 +441,12 @@ struct cam_itebd_s {
   std::unique_ptr<cam::ITEBD> d;
   cam_ctx* owner;
 };
+struct cam_unitcell_s {
+  cam::StoredBasis basis;
+  std::vector<TensorPtr> matrices;
+  std::vector<std::vector<double>> lambdas;
+  cam_ctx* owner;
+};
 struct cam_ftebd_s {
   std::unique_ptr<cam::FTEBD> d;
   cam_ctx* owner;
@@ -466,7 +473,7 @@ int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record)
   const double v[16] = {double(r.N),   r.energy,       r.energy_per_vertex, r.entropy,
                         r.truncation,  double(r.chi),  double(r.n),         r.matvecs_fp64,
                         r.matvecs_fp32, r.residual,    r.seconds,           r.heff_flops,
-                        r.eig_seconds, r.svd_seconds,  r.block_seconds,     0.0};
+                        r.eig_seconds, r.svd_seconds,  r.block_seconds,     r.fidelity};
   std::copy(v, v + 16, record);
   CAM_API_END
 }
@@ -598,6 +605,122 @@ int cam_ftebd_site(const cam_ftebd* t, int64_t v, cam_tensor** out) {
 int cam_ftebd_destroy(cam_ftebd* t) {
   CAM_API_BEGIN
   delete t;
+  CAM_API_END
+}
+
+namespace {
+cam::StoredBasis make_basis(int nrules, const int16_t* rules, int64_t d, const int16_t* charges,
+                            const double* energies) {
+  CAM_REQUIRE(nrules >= 1 && nrules <= cam::kMaxRules, "unsupported number of charge rules");
+  CAM_REQUIRE(d >= 1 && charges, "basis: no physical states");
+  cam::StoredBasis b;
+  b.rules.N.assign(rules, rules + nrules);
+  for (int64_t e = 0; e < d; ++e) b.charges.push_back(b.rules.normalise(charges + e * nrules));
+  b.energies.assign(static_cast<size_t>(d), 0.0);
+  if (energies) std::copy(energies, energies + d, b.energies.begin());
+  return b;
+}
+}  // namespace
+
+int cam_tensor_store(const cam_tensor* t, const char* path) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(t && path, "null argument");
+  set_device(t->owner);
+  cam::store_tensor(path, *t->t, t->owner->ctx.stream);
+  CAM_API_END
+}
+
+int cam_tensor_load(cam_ctx* c, const char* path, int nrules, const int16_t* rules, int64_t d,
+                    const int16_t* phys_charges, cam_tensor** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(c && path && rules && out, "null argument");
+  set_device(c);
+  const cam::StoredBasis b = make_basis(nrules, rules, d, phys_charges, nullptr);
+  *out = new cam_tensor{cam::load_tensor(path, b.rules, b.charges, c->ctx.stream), c};
+  CAM_API_END
+}
+
+int cam_unitcell_store(const char* path, int nrules, const int16_t* rules, int64_t d, const int16_t* phys_charges,
+                       const double* energies, int n, const cam_tensor* const* matrices, const double* const* lambdas,
+                       const int64_t* lambda_sizes) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(path && rules && (n == 0 || (matrices && lambdas && lambda_sizes)), "null argument");
+  CAM_REQUIRE(n >= 0, "negative unit cell size");
+  const cam::StoredBasis b = make_basis(nrules, rules, d, phys_charges, energies);
+  std::vector<TensorPtr> m;
+  std::vector<std::vector<double>> l;
+  hipStream_t s = nullptr;
+  for (int k = 0; k < n; ++k) {
+    CAM_REQUIRE(matrices[k], "null unit cell matrix");
+    if (k == 0) {
+      set_device(matrices[k]->owner);
+      s = matrices[k]->owner->ctx.stream;
+    }
+    m.push_back(matrices[k]->t);
+    l.emplace_back(lambdas[k], lambdas[k] + lambda_sizes[k]);
+  }
+  cam::store_unitcell(path, b, m, l, s);
+  CAM_API_END
+}
+
+int cam_unitcell_load(cam_ctx* c, const char* path, cam_unitcell** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(c && path && out, "null argument");
+  set_device(c);
+  auto* u = new cam_unitcell;
+  u->owner = c;
+  try {
+    cam::load_unitcell(path, u->basis, u->matrices, u->lambdas, c->ctx.stream);
+  } catch (...) {
+    delete u;
+    throw;
+  }
+  *out = u;
+  CAM_API_END
+}
+
+int cam_unitcell_basis(const cam_unitcell* u, int* nrules, int16_t* rules, int64_t* d, int16_t* charges,
+                       double* energies, int64_t cap) {
+  CAM_API_BEGIN
+  const cam::StoredBasis& b = u->basis;
+  const int nr = b.rules.n();
+  if (nrules) *nrules = nr;
+  if (d) *d = static_cast<int64_t>(b.charges.size());
+  if (rules) std::copy(b.rules.N.begin(), b.rules.N.end(), rules);
+  CAM_REQUIRE((!charges && !energies) || static_cast<int64_t>(b.charges.size()) <= cap, "capacity too small");
+  if (charges)
+    for (size_t e = 0; e < b.charges.size(); ++e) std::copy(b.charges[e].q.begin(), b.charges[e].q.begin() + nr,
+                                                            charges + e * nr);
+  if (energies) std::copy(b.energies.begin(), b.energies.end(), energies);
+  CAM_API_END
+}
+
+int cam_unitcell_size(const cam_unitcell* u, int* n) {
+  CAM_API_BEGIN
+  *n = static_cast<int>(u->matrices.size());
+  CAM_API_END
+}
+
+int cam_unitcell_matrix(const cam_unitcell* u, int k, cam_tensor** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(k >= 0 && k < static_cast<int>(u->matrices.size()), "unit cell index out of range");
+  *out = new cam_tensor{u->matrices[k], u->owner};
+  CAM_API_END
+}
+
+int cam_unitcell_lambda(const cam_unitcell* u, int k, double* values, int64_t cap, int64_t* n) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(k >= 0 && k < static_cast<int>(u->lambdas.size()), "unit cell index out of range");
+  const auto& v = u->lambdas[k];
+  if (n) *n = static_cast<int64_t>(v.size());
+  CAM_REQUIRE(static_cast<int64_t>(v.size()) <= cap || !values, "capacity too small");
+  if (values) std::copy(v.begin(), v.end(), values);
+  CAM_API_END
+}
+
+int cam_unitcell_destroy(cam_unitcell* u) {
+  CAM_API_BEGIN
+  delete u;
   CAM_API_END
 }
 

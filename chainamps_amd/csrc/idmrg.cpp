@@ -123,6 +123,24 @@ TensorPtr make_prediction(Ctx& ctx, const BSTensor& U, const std::vector<double>
   return contract(ctx, *ALambdaR, false, *LambdaLB, false, {{2, 0}});
 }
 
+double fidelity(Ctx& ctx, const std::vector<double>& values, const BSTensor& Vd, const std::vector<double>& prev) {
+  // MakePrediction's Auxiliary = diag(s') W^dagger comes from the SVD
+  // ALambdaR = reorder(diag(values) . Vd, 102, 2) = C s' W^dagger, so
+  // Auxiliary . diag(prev) = C^dagger (ALambdaR . diag(prev)) and, C being an
+  // isometry, both have the same singular values: CheckConvergence is
+  // 1 - sum of the singular values of ALambdaR . diag(prev), values only.
+  TensorPtr Vs = copy_tensor(ctx, Vd);
+  scale_along_index(ctx, *Vs, 0, values);
+  CAM_REQUIRE(static_cast<int64_t>(prev.size()) == Vs->idx[2].size, "fidelity: previous bond size mismatch");
+  scale_along_index(ctx, *Vs, 2, prev);
+  TensorPtr X = reorder(ctx, *Vs, false, {1, 0, 2}, 2);
+  std::vector<double> sv;
+  singular_values(ctx, *X, sv);
+  double sum = 0.0;
+  for (double x : sv) sum += x;
+  return 1.0 - sum;
+}
+
 IDMRG::IDMRG(Ctx& c, TensorPtr mpo, const Charge& tgt, bool conv, bool mix, int k)
     : ctx(&c), W(std::move(mpo)), target(tgt), converge(conv), mixed(mix), ncv(k) {
   CAM_REQUIRE(W->nidx() == 4, "iDMRG needs a 4-index MPO");
@@ -194,6 +212,8 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
   } catch (const Error&) {
     guess.reset();   // inconsistent bond charges: next solve starts at random
   }
+  // 1 - fidelity of the prediction (CheckConvergence, DMRG_routines.cpp:769-771)
+  rec.fidelity = fidelity(*ctx, values, *Vd, prev_lambda);
   rec.block_seconds += lap(tp);
   rec.N = size;
   rec.energy = E;

@@ -200,77 +200,6 @@ __global__ void multi_axpy_kernel(const double2* __restrict__ V, int64_t ldv, in
   }
 }
 
-// One sweep over the rows of a multi_dot slice: w_i -= sum_j V_ij c_j, then
-// (NORM = false) the partial sums of conj(V_ij) w_i for every j, or (NORM)
-// the partial sum of |w_i|^2.  The row's V values stay in registers between
-// the update and the dots.
-template <int KMAX, bool NORM>
-__global__ __launch_bounds__(256) void axpy_dot_kernel(const double2* __restrict__ V, int64_t ldv, int k,
-                                                       const double2* __restrict__ c, double2* __restrict__ w,
-                                                       int64_t n, double2* __restrict__ partial) {
-  __shared__ double2 cc[KMAX];
-  __shared__ double red_r[KMAX][8];
-  __shared__ double red_i[KMAX][8];
-  if (threadIdx.x < k) cc[threadIdx.x] = c[threadIdx.x];
-  __syncthreads();
-  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = blockIdx.x * per;
-  const int64_t hi = min(n, lo + per);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double sr[NORM ? 1 : KMAX], si[NORM ? 1 : KMAX];
-#pragma unroll
-  for (int q = 0; q < (NORM ? 1 : KMAX); ++q) { sr[q] = 0; si[q] = 0; }
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    double2 a[KMAX];
-    double r = 0, im = 0;
-#pragma unroll
-    for (int q = 0; q < KMAX; ++q) {
-      if (q < k) {
-        a[q] = V[i + q * ldv];
-        r += a[q].x * cc[q].x - a[q].y * cc[q].y;
-        im += a[q].x * cc[q].y + a[q].y * cc[q].x;
-      }
-    }
-    double2 b = w[i];
-    b.x -= r;
-    b.y -= im;
-    w[i] = b;
-    if (NORM) {
-      sr[0] += b.x * b.x + b.y * b.y;
-    } else {
-#pragma unroll
-      for (int q = 0; q < KMAX; ++q) {
-        if (q < k) {
-          sr[q] += a[q].x * b.x + a[q].y * b.y;
-          si[q] += a[q].x * b.y - a[q].y * b.x;
-        }
-      }
-    }
-  }
-  const int kk = NORM ? 1 : k;
-#pragma unroll
-  for (int q = 0; q < (NORM ? 1 : KMAX); ++q) {
-    double rr = sr[q], ii = si[q];
-    for (int o = 32; o > 0; o >>= 1) {
-      rr += __shfl_down(rr, o, 64);
-      ii += __shfl_down(ii, o, 64);
-    }
-    if (lane == 0 && q < kk) {
-      red_r[q][wave] = rr;
-      red_i[q][wave] = ii;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < kk) {
-    double rr = 0, ii = 0;
-    for (int q = 0; q < static_cast<int>(blockDim.x >> 6); ++q) {
-      rr += red_r[threadIdx.x][q];
-      ii += red_i[threadIdx.x][q];
-    }
-    partial[blockIdx.x * int64_t(kk) + threadIdx.x] = make_double2(rr, ii);
-  }
-}
-
 inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g, 8192)));
@@ -405,42 +334,6 @@ void launch_multi_axpy(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w
   hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(V), ldv, k,
                      reinterpret_cast<const double2*>(c), reinterpret_cast<double2*>(w), n, 1);
-  HIP_CHECK(hipGetLastError());
-}
-
-void launch_axpy_dot(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, cplx* partial, cplx* d,
-                     hipStream_t s) {
-  if (k > kFusedMaxK) {
-    launch_multi_axpy(V, ldv, k, c, w, n, s);
-    launch_multi_dot(V, ldv, k, w, n, partial, d, s);
-    return;
-  }
-  CAM_REQUIRE(k >= 1, "axpy_dot: k out of range");
-  const int nb = multi_dot_blocks(n);
-  hipLaunchKernelGGL((axpy_dot_kernel<kFusedMaxK, false>), dim3(nb), dim3(256), 0, s,
-                     reinterpret_cast<const double2*>(V), ldv, k, reinterpret_cast<const double2*>(c),
-                     reinterpret_cast<double2*>(w), n, reinterpret_cast<double2*>(partial));
-  HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(multi_dot_final, dim3(k), dim3(256), 0, s, reinterpret_cast<const double2*>(partial), nb, k,
-                     reinterpret_cast<double2*>(d));
-  HIP_CHECK(hipGetLastError());
-}
-
-void launch_axpy_norm(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, cplx* partial, cplx* d,
-                      hipStream_t s) {
-  if (k > kFusedMaxK) {
-    launch_multi_axpy(V, ldv, k, c, w, n, s);
-    launch_multi_dot(w, n, 1, w, n, partial, d, s);
-    return;
-  }
-  CAM_REQUIRE(k >= 1, "axpy_norm: k out of range");
-  const int nb = multi_dot_blocks(n);
-  hipLaunchKernelGGL((axpy_dot_kernel<kFusedMaxK, true>), dim3(nb), dim3(256), 0, s,
-                     reinterpret_cast<const double2*>(V), ldv, k, reinterpret_cast<const double2*>(c),
-                     reinterpret_cast<double2*>(w), n, reinterpret_cast<double2*>(partial));
-  HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(multi_dot_final, dim3(1), dim3(256), 0, s, reinterpret_cast<const double2*>(partial), nb, 1,
-                     reinterpret_cast<double2*>(d));
   HIP_CHECK(hipGetLastError());
 }
 

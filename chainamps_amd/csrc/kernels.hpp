@@ -52,17 +52,6 @@ void launch_multi_axpy(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w
 void launch_combine(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n,
                     hipStream_t s);
 int multi_dot_blocks(int64_t n);
-// Fused Gram-Schmidt sweeps (one read of V instead of two):
-//   axpy_dot:  w -= V c;  then d[j] = sum_i conj(V[i + j*ldv]) * w[i]
-//   axpy_norm: w -= V c;  then d[0] = sum_i |w[i]|^2 (as a complex with zero
-//              imaginary part)
-// partial as for launch_multi_dot; k <= kFusedMaxK, else two separate sweeps.
-constexpr int kFusedMaxK = 24;
-void launch_axpy_dot(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, cplx* partial, cplx* d,
-                     hipStream_t s);
-void launch_axpy_norm(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, cplx* partial, cplx* d,
-                      hipStream_t s);
-
 // x[i] = (a, b), a and b uniform in [-1, 1) from a counter-based splitmix64
 // stream (deterministic for a given seed)
 void launch_random_fill(cplx* x, int64_t n, uint64_t seed, hipStream_t s);

@@ -123,11 +123,14 @@ struct Lanczos {
       H.apply(col(j), w.get(), prec);
       cplx* c1 = C1.get() + j * 64;
       cplx* c2 = C2.get() + j * 64;
-      // CGS2 in three sweeps over the basis: c1 = V^H w; w -= V c1 fused
-      // with c2 = V^H w; w -= V c2 fused with |w|^2
+      // CGS2: two passes of c = V^H w, w -= V c, then |w|^2.  (A fused
+      // update-and-dot sweep reads the basis once less but, holding a row of
+      // V per thread, ran at half the bandwidth of the two separate sweeps.)
       launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c1, s);
-      launch_axpy_dot(V.get(), n, j + 1, c1, w.get(), n, partial.get(), c2, s);
-      launch_axpy_norm(V.get(), n, j + 1, c2, w.get(), n, partial.get(), Nrm.get() + j, s);
+      launch_multi_axpy(V.get(), n, j + 1, c1, w.get(), n, s);
+      launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c2, s);
+      launch_multi_axpy(V.get(), n, j + 1, c2, w.get(), n, s);
+      launch_multi_dot(w.get(), n, 1, w.get(), n, partial.get(), Nrm.get() + j, s);
       const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)));
       hipLaunchKernelGGL(lanczos_next_kernel, dim3(grid), dim3(256), 0, s,
                          reinterpret_cast<const double2*>(w.get()),

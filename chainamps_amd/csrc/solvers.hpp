@@ -50,6 +50,10 @@ void jordan_wielandt_svd(Ctx& ctx, const cplx* B, int64_t ldb, int64_t m, int64_
 void svd_truncate(Ctx& ctx, const BSTensor& theta, int64_t D, double max_trunc, TensorPtr& U,
                   TensorPtr& Vd, std::vector<double>& values, double& discarded);
 
+// The singular values MPX_matrix::SVD would keep with no truncation (D = 0,
+// no truncation weight), without computing the singular vectors.
+void singular_values(Ctx& ctx, const BSTensor& theta, std::vector<double>& values);
+
 // iTEBD::apply_and_decompose.
 void tebd_gate(Ctx& ctx, const BSTensor& G, const BSTensor& A0, const BSTensor& A1,
                const std::vector<double>& lambda0, int64_t D, double minS, TensorPtr& A0n,

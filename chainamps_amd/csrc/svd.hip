@@ -135,8 +135,10 @@ struct Group {
 
 }  // namespace
 
-void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorPtr& Uout,
-                  TensorPtr& Vdout, std::vector<double>& values, double& discarded) {
+namespace {
+
+void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorPtr& Uout, TensorPtr& Vdout,
+              std::vector<double>& values, double& discarded, bool values_only) {
   hipStream_t s = ctx.stream;
   const double eps = std::numeric_limits<double>::epsilon();
   const int r = th.numrows, N = th.nidx();
@@ -245,16 +247,18 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
     g.k = std::min(g.mg, g.ng);
     g.truncated = false;
     g.tail = 0.0;
-    g.U.alloc(g.mg * g.k);
-    g.Vh.alloc(g.k * g.ng);
+    // values only: no singular vectors (U, V^dagger are 1-element placeholders)
+    g.U.alloc(values_only ? 1 : g.mg * g.k);
+    g.Vh.alloc(values_only ? 1 : g.k * g.ng);
     dS.ensure(g.k);
     dE.ensure(std::max<int64_t>(g.k, 1));
+    const rocblas_svect sv = values_only ? rocblas_svect_none : rocblas_svect_singular;
     const rocblas_status st = rocsolver_zgesvd(
-        h, rocblas_svect_singular, rocblas_svect_singular, static_cast<rocblas_int>(g.mg),
+        h, sv, sv, static_cast<rocblas_int>(g.mg),
         static_cast<rocblas_int>(g.ng), reinterpret_cast<rocblas_double_complex*>(g.A.get()),
         static_cast<rocblas_int>(g.mg), dS.get(), reinterpret_cast<rocblas_double_complex*>(g.U.get()),
-        static_cast<rocblas_int>(g.mg), reinterpret_cast<rocblas_double_complex*>(g.Vh.get()),
-        static_cast<rocblas_int>(g.k), dE.get(), rocblas_outofplace, dinfo.get());
+        static_cast<rocblas_int>(values_only ? 1 : g.mg), reinterpret_cast<rocblas_double_complex*>(g.Vh.get()),
+        static_cast<rocblas_int>(values_only ? 1 : g.k), dE.get(), rocblas_outofplace, dinfo.get());
     CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgesvd failed");
     g.S.resize(g.k);
     int info = 0;
@@ -401,6 +405,7 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
   for (Group* g : glist) g->A.reset();
   values.clear();
   for (size_t i = 0; i < length; ++i) values.push_back(all[i].v);
+  if (values_only) return;
 
   // ---- new bond index and the U / V^dagger tensors ------------------------
   QIndex nb;
@@ -520,6 +525,19 @@ void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, Ten
   }
   Uout = U;
   Vdout = Vd;
+}
+
+}  // namespace
+
+void svd_truncate(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorPtr& Uout,
+                  TensorPtr& Vdout, std::vector<double>& values, double& discarded) {
+  svd_core(ctx, th, D, max_trunc, Uout, Vdout, values, discarded, false);
+}
+
+void singular_values(Ctx& ctx, const BSTensor& th, std::vector<double>& values) {
+  TensorPtr U, Vd;
+  double disc = 0.0;
+  svd_core(ctx, th, 0, 0.0, U, Vd, values, disc, true);
 }
 
 }  // namespace cam

@@ -210,7 +210,8 @@ int cam_bond_evolution_operator(cam_ctx* ctx, const cam_tensor* H, double timest
  * vertices; record[16] receives {N, energy, energy/N, entropy,
  * truncation, chi kept, n, matvecs fp64, matvecs fp32, residual, seconds,
  * heff flops, eigensolve seconds, SVD seconds, block update + H_eff build
- * + prediction seconds, 0}. */
+ * + prediction seconds, 1 - fidelity of the prediction (CheckConvergence,
+ * DMRG_routines.cpp:769-771, the Energies.dat column)}. */
 typedef struct cam_idmrg_s cam_idmrg;
 int cam_idmrg_create(cam_ctx* ctx, const cam_tensor* W, const int16_t* target, int converge,
                      int mixed, int ncv, cam_idmrg** out);
@@ -261,6 +262,31 @@ int cam_ftebd_evolve(cam_ftebd* t, int64_t nsteps, int64_t D, double minS, int64
 int cam_ftebd_records(const cam_ftebd* t, double* records, int64_t cap, int64_t* n);
 int cam_ftebd_site(const cam_ftebd* t, int64_t v, cam_tensor** out);
 int cam_ftebd_destroy(cam_ftebd* t);
+
+/* ---- on-disk formats (checkpoint / resume interop) --------------------------
+ * The reference's binary files.  cam_tensor_store / cam_tensor_load:
+ * MPX_matrix::store / load_MPX_matrix (MPX_matrix.cpp:219-247, 594-612) for
+ * .MPX_matrix and .MPS_matrix files; a physical index carries no charges on
+ * disk, so the loader takes the basis charges (nrules rules, d states,
+ * charges d x nrules).  The unit cell: UnitCell::store / load_UnitCell_binary
+ * (UnitCell.cpp:70-187), i.e. the basis (Basis::fprint_binary, states.cpp:
+ * 119-189: rules, charges, energies), the MPS matrices and the lambdas;
+ * loaded cells are read back through the accessors. */
+int cam_tensor_store(const cam_tensor* t, const char* path);
+int cam_tensor_load(cam_ctx* ctx, const char* path, int nrules, const int16_t* rules, int64_t d,
+                    const int16_t* phys_charges, cam_tensor** out);
+typedef struct cam_unitcell_s cam_unitcell;
+int cam_unitcell_store(const char* path, int nrules, const int16_t* rules, int64_t d, const int16_t* phys_charges,
+                       const double* energies, int n, const cam_tensor* const* matrices, const double* const* lambdas,
+                       const int64_t* lambda_sizes);
+int cam_unitcell_load(cam_ctx* ctx, const char* path, cam_unitcell** out);
+/* rules (nrules entries), charges (d x nrules) and energies (d) with capacity cap states */
+int cam_unitcell_basis(const cam_unitcell* u, int* nrules, int16_t* rules, int64_t* d, int16_t* charges,
+                       double* energies, int64_t cap);
+int cam_unitcell_size(const cam_unitcell* u, int* n);
+int cam_unitcell_matrix(const cam_unitcell* u, int k, cam_tensor** out);
+int cam_unitcell_lambda(const cam_unitcell* u, int k, double* values, int64_t cap, int64_t* n);
+int cam_unitcell_destroy(cam_unitcell* u);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * Vector-Jacobian product of E = <psi|H_eff|psi>/<psi|psi>:

@@ -209,6 +209,13 @@ class MPX_matrix {
     check(cam_tensor_rescale(t_.get(), factor.real(), factor.imag()));
     return *this;
   }
+  /* MPX_matrix::store (MPX_matrix.cpp:238-247): the reference's binary
+   * .MPX_matrix / .MPS_matrix file.  Returns false on success, as the
+   * reference's does. */
+  bool store(const std::string& filename) const {
+    check(cam_tensor_store(t_.get(), filename.c_str()));
+    return false;
+  }
   /* MPX_matrix::SVD (MPX_matrix.hpp:116). */
   MPXDecomposition SVD(size_t bond_dimension = 0, double min_s_val = 0.0) const;
 
@@ -276,6 +283,18 @@ inline MPX_matrix contract(const MPX_matrix& A, bool conjA, const MPX_matrix& B,
 inline MPX_matrix contract(const MPX_matrix& A, bool conjA, const MPX_matrix& B, bool conjB,
                            const MPXPair& contractidx) {
   return contract(A, conjA, B, conjB, std::vector<MPXPair>(1, contractidx));
+}
+
+/* load_MPX_matrix (MPX_matrix.cpp:594-612): physical indices take the
+ * charges of the basis states `spectrum` (one State per basis state). */
+inline MPX_matrix load_MPX_matrix(const std::string& filename, const std::vector<short>& rules,
+                                  const StateArray& spectrum) {
+  std::vector<int16_t> r(rules.begin(), rules.end()), q;
+  for (const State& s : spectrum) q.insert(q.end(), s.begin(), s.end());
+  cam_tensor* t = nullptr;
+  check(cam_tensor_load(Context::get(), filename.c_str(), static_cast<int>(r.size()), r.data(),
+                        static_cast<int64_t>(spectrum.size()), q.data(), &t));
+  return MPX_matrix(t);
 }
 
 /* ajaj::reorder (MPX_matrix.cpp:630-638). */

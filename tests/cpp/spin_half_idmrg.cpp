@@ -98,6 +98,21 @@ int main() {
   for (double v : dec.Values) s2 += v * v;
   expect(std::fabs(s2 - fro2) < 1e-13 * fro2, "SVD sum s^2 = |AB|_F^2", s2, fro2);
 
+  // store / load_MPX_matrix round trip of the MPO in the reference's format
+  {
+    const std::string path = "/tmp/cam_spin_half_H.MPO_matrix";
+    H.store(path);
+    const MPX_matrix H2 = load_MPX_matrix(path, rules, phys);
+    std::vector<long long> p1, i1, p2, i2;
+    std::vector<C> x1, x2;
+    H.to_csc(p1, i1, x1);
+    H2.to_csc(p2, i2, x2);
+    std::remove(path.c_str());
+    const bool same = p1 == p2 && i1 == i2 && x1 == x2 && H2.NumRowIndices() == H.NumRowIndices();
+    expect(same, "store / load_MPX_matrix round trip (nnz)", static_cast<double>(x2.size()),
+           static_cast<double>(x1.size()));
+  }
+
   // finite-chain TEBD (TEBD::evolve) from the all-up product state of N = 4
   // sites: it is an eigenstate of every bond Hamiltonian (odd bonds carry the
   // field of both sites: -1 + 2e-5, even bonds -1), so each gate only adds a

@@ -34,6 +34,8 @@ def test_idmrg_heisenberg_vs_oracle(ctx):
         assert g["chi"] == r["chi"], (step, g["chi"], r["chi"])
         assert abs(g["entropy"] - r["entropy"]) <= 1e-9, (step, g["entropy"], r["entropy"])
         assert np.allclose(np.sort(dev.values()), np.sort(ref.values), atol=1e-9)
+        # 1 - fidelity of the prediction (CheckConvergence), the Energies.dat column
+        assert abs(g["fidelity"] - r["fidelity"]) <= 1e-8, (step, g["fidelity"], r["fidelity"])
 
 
 def test_idmrg_spin_half_fixture(ctx):

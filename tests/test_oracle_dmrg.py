@@ -147,3 +147,16 @@ def test_heff_hermitian_for_physical_model():
                                     dmrg.make_right_block(idm.RB, W, idm.Vd))
     H = comp.dense_matrix()
     assert np.allclose(H, H.conj().T, atol=1e-12)
+
+
+def test_prediction_fidelity_limits():
+    """CheckConvergence (DMRG_routines.cpp:769-771): while the bond is not
+    truncated the prediction is exact and 1 - fidelity is 0; once it is
+    truncated 1 - fidelity is positive and falls as iDMRG converges to its
+    fixed point."""
+    rules, phys, W = models.heisenberg_mpo()
+    ref = dmrg.IDMRG(W, converge=True)
+    fids = [ref.grow(16, 1e-12)["fidelity"] for _ in range(14)]
+    assert abs(fids[0]) < 1e-12 and abs(fids[1]) < 1e-12
+    assert all(f > 0 for f in fids[2:])
+    assert fids[-1] < 0.2 * fids[2]
