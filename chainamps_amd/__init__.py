@@ -8,9 +8,9 @@ first use (``load_library``).
 """
 from .capi import (CamError, Context, Tensor, Heff, IDMRG, ITEBD, FTEBD, contract, reorder, svd_truncate,
                    tebd_gate, extract_mpo_block, bond_hamiltonian, bond_evolution_operator,
-                   store_unitcell, load_unitcell, load_library, library_path, CAM_FP64, CAM_FP32)
+                   store_unitcell, load_unitcell, UnitCell, load_library, library_path, CAM_FP64, CAM_FP32)
 
 __all__ = ["CamError", "Context", "Tensor", "Heff", "IDMRG", "ITEBD", "FTEBD", "contract", "reorder", "svd_truncate",
-           "store_unitcell", "load_unitcell",
+           "store_unitcell", "load_unitcell", "UnitCell",
            "tebd_gate", "extract_mpo_block", "bond_hamiltonian", "bond_evolution_operator",
            "load_library", "library_path", "CAM_FP64", "CAM_FP32"]

@@ -476,6 +476,13 @@ class IDMRG:
                                      C.byref(h)))
         self.h = h
 
+    def unitcell(self):
+        """UnitCell(getCentralDecomposition(), getPreviousLambda()) of the
+        last growth step (iDMRG_DRV.cpp:70), as a UnitCell."""
+        h = _p()
+        _check(_LIB.cam_idmrg_unitcell(self.h, C.byref(h)))
+        return UnitCell(self.ctx, h)
+
     def grow(self, chi, truncation=0.0):
         rec = np.zeros(16)
         _check(_LIB.cam_idmrg_grow(self.h, int(chi), float(truncation), _ptr(rec, C.c_double)))
@@ -675,3 +682,101 @@ def load_unitcell(ctx, path):
         return rules, ch, en, mats, lams
     finally:
         _LIB.cam_unitcell_destroy(h)
+
+
+_SIGS_ORTHO = {
+    "cam_unitcell_create": [_p, C.c_int, C.POINTER(_p), C.POINTER(_dp), _i64p, C.POINTER(_p)],
+    "cam_unitcell_from_decomposition": [_p, _p, _dp, C.c_int64, _p, _dp, C.c_int64, C.POINTER(_p)],
+    "cam_idmrg_unitcell": [_p, C.POINTER(_p)],
+    "cam_transfer_leading": [_p, C.c_int, C.c_double, _dp, _dp],
+    "cam_orthogonalise": [_p, C.POINTER(_p), _dp],
+    "cam_itwovertex_energy": [_p, _p, _dp, _dp],
+}
+_SIGS.update(_SIGS_ORTHO)
+
+ORTHO_FIELDS = ("left_re", "left_im", "right_re", "right_im", "entropy", "chi", "left_matvecs", "right_matvecs",
+                "left_residual", "right_residual", "kept", "seconds")
+
+
+class UnitCell:
+    """A unit cell on the device (UnitCell): MPS matrices [s, a_l, a_r] and
+    the lambda on the left of each."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    @classmethod
+    def create(cls, ctx, matrices, lambdas):
+        """UnitCell(spectrum, matrices, lambdas) (UnitCell.hpp:41)."""
+        n = len(matrices)
+        hs = (_p * n)(*[m.h for m in matrices])
+        ls = [np.ascontiguousarray(l, dtype=np.float64) for l in lambdas]
+        lp = (_dp * n)(*[_ptr(l, C.c_double) for l in ls])
+        sz = np.array([len(l) for l in ls], np.int64)
+        h = _p()
+        _check(_LIB.cam_unitcell_create(ctx.h, n, hs, lp, _ptr(sz, C.c_int64), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_decomposition(cls, U, values, Vd, previous):
+        """UnitCell(MPSDecomposition, PreviousLambda) (UnitCell.hpp:42-48)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        p = np.ascontiguousarray(previous, dtype=np.float64)
+        h = _p()
+        _check(_LIB.cam_unitcell_from_decomposition(U.ctx.h, U.h, _ptr(v, C.c_double), len(v), Vd.h,
+                                                    _ptr(p, C.c_double), len(p), C.byref(h)))
+        return cls(U.ctx, h)
+
+    def size(self):
+        n = C.c_int()
+        _check(_LIB.cam_unitcell_size(self.h, C.byref(n)))
+        return n.value
+
+    def matrix(self, k) -> Tensor:
+        t = _p()
+        _check(_LIB.cam_unitcell_matrix(self.h, int(k), C.byref(t)))
+        return Tensor(self.ctx, t)
+
+    def lam(self, k):
+        m = C.c_int64()
+        _check(_LIB.cam_unitcell_lambda(self.h, int(k), None, 0, C.byref(m)))
+        v = np.zeros(max(m.value, 1))
+        _check(_LIB.cam_unitcell_lambda(self.h, int(k), _ptr(v, C.c_double), len(v), C.byref(m)))
+        return v[:m.value]
+
+    def transfer_leading(self, side, tol=0.0):
+        """Leading eigenvalue of the left (side 0) or right (side 1)
+        transfer map (TransferMatrixComponents::LeftED / RightED)."""
+        val = np.zeros(2)
+        info = np.zeros(4)
+        _check(_LIB.cam_transfer_leading(self.h, int(side), float(tol), _ptr(val, C.c_double),
+                                         _ptr(info, C.c_double)))
+        return complex(val[0], val[1]), dict(matvecs=int(info[0]), restarts=int(info[1]), residual=info[2],
+                                             converged=bool(info[3]))
+
+    def orthogonalise(self):
+        """Orthogonalise (measurement.cpp:180-246): (new cell, info)."""
+        h = _p()
+        info = np.zeros(len(ORTHO_FIELDS))
+        _check(_LIB.cam_orthogonalise(self.h, C.byref(h), _ptr(info, C.c_double)))
+        return UnitCell(self.ctx, h), dict(zip(ORTHO_FIELDS, info.tolist()))
+
+    def energy(self, W: Tensor):
+        """iTwoVertexEnergy (measurement.cpp:371-397): (energy per vertex,
+        [on-vertex, bond in the cell, bond to the next cell])."""
+        e = np.zeros(2)
+        t = np.zeros(6)
+        _check(_LIB.cam_itwovertex_energy(W.h, self.h, _ptr(e, C.c_double), _ptr(t, C.c_double)))
+        return complex(e[0], e[1]), [complex(t[2 * k], t[2 * k + 1]) for k in range(3)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            _LIB.cam_unitcell_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,6 +11,7 @@
 #include "itebd.hpp"
 #include "ftebd.hpp"
 #include "storage.hpp"
+#include "ortho.hpp"
 #include <memory>
 
 using namespace cam;
@@ -721,6 +722,117 @@ int cam_unitcell_lambda(const cam_unitcell* u, int k, double* values, int64_t ca
 int cam_unitcell_destroy(cam_unitcell* u) {
   CAM_API_BEGIN
   delete u;
+  CAM_API_END
+}
+
+// ---------------------------------------------------------------------------
+// Unit cells on the device, orthogonalisation and the energy per vertex
+// ---------------------------------------------------------------------------
+
+static cam_unitcell* new_cell(cam_ctx* c, std::vector<TensorPtr> m, std::vector<std::vector<double>> l) {
+  CAM_REQUIRE(!m.empty() && m.size() == l.size(), "Invalid Unit Cell: matrices and lambdas differ in number");
+  auto* u = new cam_unitcell;
+  u->owner = c;
+  u->basis.rules = m.front()->rules;
+  CAM_REQUIRE(m.front()->nidx() >= 1 && m.front()->idx[0].physical, "unit cell matrix without a physical index");
+  u->basis.charges = m.front()->idx[0].charges;
+  u->basis.energies.assign(u->basis.charges.size(), 0.0);
+  u->matrices = std::move(m);
+  u->lambdas = std::move(l);
+  return u;
+}
+
+int cam_unitcell_create(cam_ctx* c, int n, const cam_tensor* const* matrices, const double* const* lambdas,
+                        const int64_t* lambda_sizes, cam_unitcell** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(c && out && n >= 1 && matrices && lambdas && lambda_sizes, "null argument");
+  std::vector<TensorPtr> m;
+  std::vector<std::vector<double>> l;
+  for (int k = 0; k < n; ++k) {
+    CAM_REQUIRE(matrices[k] && (lambdas[k] || lambda_sizes[k] == 0), "null unit cell entry");
+    m.push_back(matrices[k]->t);
+    l.emplace_back(lambdas[k], lambdas[k] + lambda_sizes[k]);
+  }
+  *out = new_cell(c, std::move(m), std::move(l));
+  CAM_API_END
+}
+
+int cam_unitcell_from_decomposition(cam_ctx* c, const cam_tensor* U, const double* values, int64_t nvalues,
+                                    const cam_tensor* Vd, const double* previous, int64_t nprevious,
+                                    cam_unitcell** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(c && U && Vd && values && previous && out, "null argument");
+  set_device(c);
+  std::vector<double> v(values, values + nvalues), p(previous, previous + nprevious);
+  TensorPtr A1 = cam::unit_cell_second(c->ctx, *Vd->t, v, p);
+  *out = new_cell(c, {U->t, A1}, {p, v});
+  CAM_API_END
+}
+
+int cam_idmrg_unitcell(cam_idmrg* g, cam_unitcell** out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(g && out, "null argument");
+  CAM_REQUIRE(g->d->U && g->d->Vd, "iDMRG: no decomposition yet (grow first)");
+  set_device(g->owner);
+  cam::IDMRG& d = *g->d;
+  TensorPtr A1 = cam::unit_cell_second(g->owner->ctx, *d.Vd, d.values, d.prev_lambda);
+  *out = new_cell(g->owner, {d.U, A1}, {d.prev_lambda, d.values});
+  CAM_API_END
+}
+
+int cam_transfer_leading(const cam_unitcell* u, int side, double tol, double* value, double* info) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(u && value, "null argument");
+  set_device(u->owner);
+  const cam::LeadingEig e = cam::transfer_leading(u->owner->ctx, u->matrices, side, tol > 0.0 ? tol : 1e-14, nullptr);
+  value[0] = e.value.re;
+  value[1] = e.value.im;
+  if (info) {
+    info[0] = e.matvecs;
+    info[1] = e.restarts;
+    info[2] = e.residual;
+    info[3] = e.converged ? 1.0 : 0.0;
+  }
+  CAM_API_END
+}
+
+int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(u && out, "null argument");
+  CAM_REQUIRE(u->matrices.size() == 2, "Orthogonalise: two-vertex unit cells only");
+  set_device(u->owner);
+  TensorPtr A0n, A1n;
+  std::vector<double> l0;
+  cam::OrthoInfo oi;
+  cam::orthogonalise(u->owner->ctx, u->matrices[0], u->matrices[1], A0n, A1n, l0, oi);
+  cam_unitcell* r = new_cell(u->owner, {A0n, A1n}, {l0, u->lambdas[1]});
+  r->basis = u->basis;
+  *out = r;
+  if (info) {
+    const double v[12] = {oi.left.value.re, oi.left.value.im, oi.right.value.re, oi.right.value.im,
+                          oi.entropy, static_cast<double>(l0.size()),
+                          static_cast<double>(oi.left.matvecs), static_cast<double>(oi.right.matvecs),
+                          oi.left.residual, oi.right.residual, static_cast<double>(oi.kept_left + oi.kept_right),
+                          oi.seconds};
+    std::copy(v, v + 12, info);
+  }
+  CAM_API_END
+}
+
+int cam_itwovertex_energy(const cam_tensor* W, const cam_unitcell* u, double* energy, double* terms) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(W && u && energy, "null argument");
+  CAM_REQUIRE(u->matrices.size() == 2 && u->lambdas.size() == 2, "Only two vertex basis accepted for now");
+  set_device(u->owner);
+  cplx t[3];
+  const cplx e = cam::i_two_vertex_energy(u->owner->ctx, *W->t, u->matrices[0], u->matrices[1], u->lambdas[0], t);
+  energy[0] = e.re;
+  energy[1] = e.im;
+  if (terms)
+    for (int k = 0; k < 3; ++k) {
+      terms[2 * k] = t[k].re;
+      terms[2 * k + 1] = t[k].im;
+    }
   CAM_API_END
 }
 

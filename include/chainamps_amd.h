@@ -288,6 +288,46 @@ int cam_unitcell_matrix(const cam_unitcell* u, int k, cam_tensor** out);
 int cam_unitcell_lambda(const cam_unitcell* u, int k, double* values, int64_t cap, int64_t* n);
 int cam_unitcell_destroy(cam_unitcell* u);
 
+/* ---- unit cells, orthogonalisation and the energy per vertex -------------
+ * cam_unitcell_create: UnitCell(spectrum, matrices, lambdas) (UnitCell.hpp:41)
+ * over device tensors [s (physical), a_l (in), a_r (out)] (references, not
+ * copies); the basis energies of a cell built on the device are zero.
+ * cam_unitcell_from_decomposition: UnitCell(MPSDecomposition, PreviousLambda)
+ * (UnitCell.hpp:42-48): [U, diag(values) . Vd . diag(previous)^-1 reordered
+ * (phys, in, out)], lambdas [previous, values].
+ * cam_idmrg_unitcell: the same from an iDMRG object's central decomposition
+ * and previous lambda (iDMRG_DRV.cpp:70, getCentralDecomposition /
+ * getPreviousLambda).
+ * cam_transfer_leading: TransferMatrixComponents(cell, 1, Identity).LeftED
+ * (side 0) or .RightED (side 1) with NUMEVALS = 1 and LARGESTMAGNITUDE
+ * (measurement.cpp:405-629): the leading eigenvalue value[2] (re, im) of the
+ * cell's transfer map, by a device Krylov-Schur iteration (Krylov dimension
+ * 20 as ARPACK's ncv, at most 50 restarts, Ritz residual <= tol |value|, tol
+ * <= 0 for 1e-14); info[4] = {matvecs, restarts, relative residual,
+ * converged}.  The two boundary indices of the cell must carry the same
+ * charges (cells of a zero-target iDMRG and of iTEBD do).
+ * cam_orthogonalise: Orthogonalise(const UnitCell&) (measurement.cpp:
+ * 180-246) of a two-vertex cell: a new cell [U^H X A0 / sqrt(l), A1 X^-1 U]
+ * with lambdas [new boundary values (unit square sum), the input's second
+ * lambda]; an error (the reference's empty cell) when a leading eigenvalue
+ * has |Im| >= 100 eps |Re| or SqrtDR meets eigenvalues of both signs.
+ * info[12] = {left value re, im, right value re, im, entropy, new bond
+ * dimension, left matvecs, right matvecs, left residual, right residual,
+ * kept SqrtDR vectors (left + right), seconds}.
+ * cam_itwovertex_energy: iTwoVertexEnergy(H_MPO, Ortho) (measurement.cpp:
+ * 371-397) of an orthogonalised two-vertex cell: energy[2] (re, im) per
+ * vertex; terms[6] = (on-vertex, bond inside the cell, bond to the next
+ * cell) as (re, im) pairs, before the factor 1/2. */
+int cam_unitcell_create(cam_ctx* ctx, int n, const cam_tensor* const* matrices, const double* const* lambdas,
+                        const int64_t* lambda_sizes, cam_unitcell** out);
+int cam_unitcell_from_decomposition(cam_ctx* ctx, const cam_tensor* U, const double* values, int64_t nvalues,
+                                    const cam_tensor* Vd, const double* previous, int64_t nprevious,
+                                    cam_unitcell** out);
+int cam_idmrg_unitcell(cam_idmrg* g, cam_unitcell** out);
+int cam_transfer_leading(const cam_unitcell* u, int side, double tol, double* value, double* info);
+int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info);
+int cam_itwovertex_energy(const cam_tensor* W, const cam_unitcell* u, double* energy, double* terms);
+
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * Vector-Jacobian product of E = <psi|H_eff|psi>/<psi|psi>:
  * dpsi (host, n complex, reference order) = dE/dconj(psi) and
