@@ -143,6 +143,34 @@ def optimal_order_flops_dense(LB, W, RB, comp):
     return flops
 
 
+def orthogonalise_step(dm, W):
+    """The iDMRG driver's measurement after a growth step (iDMRG_DRV.cpp:
+    64-82): the central unit cell, its left transfer-matrix fixed point alone
+    (matvec rate), then Orthogonalise and iTwoVertexEnergy."""
+    cell = dm.unitcell()
+    t0 = time.perf_counter()
+    val, tinfo = cell.transfer_leading(0)
+    t_lead = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ocell, info = cell.orthogonalise()
+    t_ortho = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e, _ = ocell.energy(W)
+    t_energy = time.perf_counter() - t0
+    lam = ocell.lam(0)
+    out = {"left_value": round(val.real, 12), "left_matvecs": tinfo["matvecs"], "left_restarts": tinfo["restarts"],
+           "left_seconds": round(t_lead, 4), "ms_per_transfer_matvec": round(1e3 * t_lead / max(tinfo["matvecs"], 1), 3),
+           "orthogonalise_seconds": round(t_ortho, 4), "energy_seconds": round(t_energy, 4),
+           "matvecs": [int(info["left_matvecs"]), int(info["right_matvecs"])], "new_chi": int(info["chi"]),
+           "entropy": round(info["entropy"], 10), "energy_per_vertex": round(e.real, 12),
+           "boundary_lambda_max": round(float(lam.max()), 12)}
+    print(f"# orthogonalise chi={out['new_chi']} {t_ortho:.2f} s, energy {t_energy:.2f} s, "
+          f"transfer matvec {out['ms_per_transfer_matvec']} ms", file=sys.stderr, flush=True)
+    ocell.close()
+    cell.close()
+    return out
+
+
 def idmrg_steps(ctx, cam, workloads, a):
     """Times cam.IDMRG growth steps on the d-level coupled-chain MPO with
     bond dimension cap chi, FP64 and FP32-mixed; the last step is the one at
@@ -159,6 +187,8 @@ def idmrg_steps(ctx, cam, workloads, a):
                                    "svd_seconds", "block_seconds", "matvecs_fp64", "matvecs_fp32")})
             print(f"# idmrg {name} step N={r['N']} chi={r['chi']} n={r['n']} {r['seconds']:.2f} s", file=sys.stderr,
                   flush=True)
+        if not mixed:
+            out["orthogonalise"] = orthogonalise_step(dm, W)
         dm.close()
         out[name] = steps
     out["last_step_seconds"] = {k: out[k][-1]["seconds"] for k in ("fp64", "mixed")}

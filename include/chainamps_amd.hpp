@@ -482,6 +482,8 @@ class TEBD {
   MPXInt n_ = 0;
 };
 
+class UnitCell;
+
 /* iDMRG growth (iDMRG::run / SuperBlock::grow_two_vertex). */
 struct iDMRGStep {
   MPXInt N = 0, chi = 0, n = 0;
@@ -497,6 +499,9 @@ class iDMRG {
       if (p) cam_idmrg_destroy(p);
     });
   }
+  /* UnitCell(getCentralDecomposition(), getPreviousLambda()) of the last
+   * growth step (iDMRG_DRV.cpp:70). */
+  inline UnitCell CentralUnitCell() const;
   iDMRGStep grow(MPXInt chi, double truncation = 0.0) {
     double r[16];
     check(cam_idmrg_grow(d_.get(), chi, truncation, r));
@@ -515,6 +520,81 @@ class iDMRG {
  private:
   std::shared_ptr<cam_idmrg> d_;
 };
+
+/* UnitCell (UnitCell.hpp:29-66) on the device: Matrices [s, a_l, a_r] and
+ * the lambda on the left of each. */
+class UnitCell {
+ public:
+  std::vector<MPX_matrix> Matrices;
+  std::vector<std::vector<double>> Lambdas;
+  UnitCell() = default;
+  UnitCell(std::vector<MPX_matrix> m, std::vector<std::vector<double>> l) : Matrices(std::move(m)), Lambdas(std::move(l)) {}
+  size_t size() const { return Matrices.size(); }
+  /* UnitCell::Entropy (UnitCell.cpp:98-106): entropy of Lambdas[0]. */
+  double Entropy() const {
+    if (Lambdas.empty()) return -1.0;
+    double s = 0.0;
+    for (double v : Lambdas[0])
+      if (v > 0.0) s -= v * v * std::log(v * v);
+    return s;
+  }
+  /* handle over the same device tensors (cam_unitcell_create) */
+  std::shared_ptr<cam_unitcell> handle() const {
+    std::vector<const cam_tensor*> h;
+    std::vector<const double*> l;
+    std::vector<int64_t> n;
+    for (const MPX_matrix& m : Matrices) h.push_back(m.handle());
+    for (const auto& v : Lambdas) {
+      l.push_back(v.data());
+      n.push_back(static_cast<int64_t>(v.size()));
+    }
+    cam_unitcell* u = nullptr;
+    check(cam_unitcell_create(Context::get(), static_cast<int>(h.size()), h.data(), l.data(), n.data(), &u));
+    return std::shared_ptr<cam_unitcell>(u, [](cam_unitcell* p) {
+      if (p) cam_unitcell_destroy(p);
+    });
+  }
+  static UnitCell FromHandle(cam_unitcell* u) {
+    std::shared_ptr<cam_unitcell> keep(u, [](cam_unitcell* p) {
+      if (p) cam_unitcell_destroy(p);
+    });
+    int n = 0;
+    check(cam_unitcell_size(u, &n));
+    UnitCell c;
+    for (int k = 0; k < n; ++k) {
+      cam_tensor* t = nullptr;
+      check(cam_unitcell_matrix(u, k, &t));
+      c.Matrices.push_back(MPX_matrix(t));
+      int64_t len = 0;
+      check(cam_unitcell_lambda(u, k, nullptr, 0, &len));
+      std::vector<double> v(static_cast<size_t>(len));
+      check(cam_unitcell_lambda(u, k, v.data(), len, &len));
+      c.Lambdas.push_back(std::move(v));
+    }
+    return c;
+  }
+};
+
+inline UnitCell iDMRG::CentralUnitCell() const {
+  cam_unitcell* u = nullptr;
+  check(cam_idmrg_unitcell(d_.get(), &u));
+  return UnitCell::FromHandle(u);
+}
+
+/* Orthogonalise (measurement.cpp:180-246) of a two-vertex cell. */
+inline UnitCell Orthogonalise(const UnitCell& C) {
+  cam_unitcell* out = nullptr;
+  check(cam_orthogonalise(C.handle().get(), &out, nullptr));
+  return UnitCell::FromHandle(out);
+}
+
+/* iTwoVertexEnergy (measurement.cpp:371-397): energy per vertex of an
+ * orthogonalised two-vertex cell. */
+inline std::complex<double> iTwoVertexEnergy(const MPX_matrix& H_MPO, const UnitCell& Ortho) {
+  double e[2];
+  check(cam_itwovertex_energy(H_MPO.handle(), Ortho.handle().get(), e, nullptr));
+  return {e[0], e[1]};
+}
 
 }  // namespace cam_ajaj
 

@@ -4,8 +4,10 @@
 // Pauli operators, U(1) charge = 2 Sz, iDMRG growth with chi = 50 and the
 // reference's growth-step eigensolver setting.  The README's growth
 // energies -1, -3, -5, -7, -9 and entanglement entropies must come out.
-// Also exercises contract / reorder / SVD / TwoVertexComponents::HED and the
-// finite-chain TEBD on a product eigenstate (exact phase).
+// Also exercises contract / reorder / SVD / TwoVertexComponents::HED, the
+// finite-chain TEBD on a product eigenstate (exact phase), and the iDMRG
+// driver's Orthogonalise + iTwoVertexEnergy on the antiferromagnet (Bethe
+// energy per vertex).
 //
 // Exit status 0 on success; prints one line per check.
 #define CAM_AJAJ_THROW
@@ -72,6 +74,34 @@ int main() {
     expect(std::fabs(r.Energy - energy[step]) < 1e-10, what, r.Energy, energy[step]);
     std::snprintf(what, sizeof what, "iDMRG N=%lld entropy", r.N);
     expect(std::fabs(r.Entropy - entropy[step]) < 1e-9, what, r.Entropy, entropy[step]);
+  }
+
+  // the iDMRG driver's measurement step (iDMRG_DRV.cpp:64-82): Orthogonalise
+  // the central unit cell and take iTwoVertexEnergy.  For the
+  // antiferromagnet H = sum sigma.sigma (bond couplings +2, +2, +1 on
+  // s+ s-, s- s+, sz sz, no field) the energy per vertex converges to
+  // the Bethe value 1 - 4 ln 2 (4 (1/4 - ln 2) with Pauli operators).
+  {
+    std::vector<C> Wa(Wd.size(), 0.0);
+    auto at2 = [&](int sp, int bl, int s, int br) -> C& { return Wa[sp + d * (bl + w * (s + d * br))]; };
+    const double Ja[3] = {2.0, 2.0, 1.0};
+    for (int sp = 0; sp < d; ++sp)
+      for (int s = 0; s < d; ++s) {
+        at2(sp, 0, s, 0) = op(0, sp, s);
+        at2(sp, w - 1, s, w - 1) = op(0, sp, s);
+        for (int c = 0; c < 3; ++c) {
+          at2(sp, w - 1, s, 1 + c) = Ja[c] * op(A[c], sp, s);
+          at2(sp, 1 + c, s, 0) = op(B[c], sp, s);
+        }
+      }
+    const MPX_matrix Ha = MPX_matrix::FromDense(rules, widx, 2, Wa);
+    iDMRG af(Ha, State{0}, /*converge=*/true);
+    for (int step = 0; step < 40; ++step) af.grow(24);
+    const UnitCell Ortho = Orthogonalise(af.CentralUnitCell());
+    const double e = iTwoVertexEnergy(Ha, Ortho).real();
+    const double bethe = 1.0 - 4.0 * std::log(2.0);
+    expect(Ortho.size() == 2 && Ortho.Entropy() > 0.0, "Orthogonalise cell", Ortho.Entropy(), 0.0);
+    expect(std::fabs(e - bethe) < 2e-4, "iTwoVertexEnergy antiferromagnet", e, bethe);
   }
 
   // contract / reorder / SVD on a two-site product: M = a (x) b over a
