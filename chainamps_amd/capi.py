@@ -688,7 +688,8 @@ _SIGS_ORTHO = {
     "cam_unitcell_create": [_p, C.c_int, C.POINTER(_p), C.POINTER(_dp), _i64p, C.POINTER(_p)],
     "cam_unitcell_from_decomposition": [_p, _p, _dp, C.c_int64, _p, _dp, C.c_int64, C.POINTER(_p)],
     "cam_idmrg_unitcell": [_p, C.POINTER(_p)],
-    "cam_transfer_leading": [_p, C.c_int, C.c_double, _dp, _dp],
+    "cam_transfer_apply": [_p, C.c_int, _dp, _dp],
+    "cam_transfer_leading": [_p, C.c_int, C.c_double, C.c_int, _dp, _dp],
     "cam_orthogonalise": [_p, C.POINTER(_p), _dp],
     "cam_itwovertex_energy": [_p, _p, _dp, _dp],
 }
@@ -745,12 +746,21 @@ class UnitCell:
         _check(_LIB.cam_unitcell_lambda(self.h, int(k), _ptr(v, C.c_double), len(v), C.byref(m)))
         return v[:m.value]
 
-    def transfer_leading(self, side, tol=0.0):
+    def transfer_apply(self, side, vec):
+        """Left/RightComponentsMultiply on the allowed entries in the
+        reference's order (c outer, r inner)."""
+        x = np.ascontiguousarray(vec, dtype=np.complex128)
+        y = np.zeros_like(x)
+        _check(_LIB.cam_transfer_apply(self.h, int(side), _ptr(x.view(np.float64), C.c_double),
+                                       _ptr(y.view(np.float64), C.c_double)))
+        return y
+
+    def transfer_leading(self, side, tol=0.0, ncv=0):
         """Leading eigenvalue of the left (side 0) or right (side 1)
         transfer map (TransferMatrixComponents::LeftED / RightED)."""
         val = np.zeros(2)
         info = np.zeros(4)
-        _check(_LIB.cam_transfer_leading(self.h, int(side), float(tol), _ptr(val, C.c_double),
+        _check(_LIB.cam_transfer_leading(self.h, int(side), float(tol), int(ncv), _ptr(val, C.c_double),
                                          _ptr(info, C.c_double)))
         return complex(val[0], val[1]), dict(matvecs=int(info[0]), restarts=int(info[1]), residual=info[2],
                                              converged=bool(info[3]))

@@ -780,11 +780,22 @@ int cam_idmrg_unitcell(cam_idmrg* g, cam_unitcell** out) {
   CAM_API_END
 }
 
-int cam_transfer_leading(const cam_unitcell* u, int side, double tol, double* value, double* info) {
+int cam_transfer_apply(const cam_unitcell* u, int side, const double* in, double* out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(u && in && out, "null argument");
+  set_device(u->owner);
+  cam::transfer_apply(u->owner->ctx, u->matrices, side, reinterpret_cast<const cplx*>(in),
+                      reinterpret_cast<cplx*>(out));
+  CAM_API_END
+}
+
+int cam_transfer_leading(const cam_unitcell* u, int side, double tol, int ncv, double* value, double* info) {
   CAM_API_BEGIN
   CAM_REQUIRE(u && value, "null argument");
+  CAM_REQUIRE(ncv <= 0 || (ncv >= 2 && ncv <= 63), "transfer matrix: ncv must be 2..63 (or <= 0 for 20)");
   set_device(u->owner);
-  const cam::LeadingEig e = cam::transfer_leading(u->owner->ctx, u->matrices, side, tol > 0.0 ? tol : 1e-14, nullptr);
+  const cam::LeadingEig e =
+      cam::transfer_leading(u->owner->ctx, u->matrices, side, tol > 0.0 ? tol : 1e-14, nullptr, ncv);
   value[0] = e.value.re;
   value[1] = e.value.im;
   if (info) {

@@ -77,6 +77,33 @@ __global__ void hermitian_complete_kernel(const double2* __restrict__ x, double2
   }
 }
 
+// In place: every n_s x n_s block s (blockIdx.y) becomes its Hermitian part
+// (x + x^H) / 2; each off-diagonal pair is owned by one thread.
+__global__ void hermitian_part_kernel(double2* __restrict__ x, const int64_t* __restrict__ off,
+                                      const int64_t* __restrict__ nn) {
+  const int s = blockIdx.y;
+  const int64_t n = nn[s], base = off[s];
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n * n;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = e % n, j = e / n;
+    if (i > j) continue;
+    if (i == j) {
+      x[base + e].y = 0.0;
+      continue;
+    }
+    const double2 a = x[base + e], b = x[base + j + i * n];
+    const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    x[base + e] = h;
+    x[base + j + i * n] = make_double2(h.x, -h.y);
+  }
+}
+
+// c[i].im = 0 for i < k (real inner products on the Hermitian subspace)
+__global__ void real_part_kernel(double2* __restrict__ c, int k) {
+  const int i = threadIdx.x;
+  if (i < k) c[i].y = 0.0;
+}
+
 // From the eigenvectors V (n x n, column-major) of one block and its kept
 // columns c0 .. c0+k-1 with roots r_i = sqrt|w_i|:
 //   X (k x n) = diag(r) V_kept^H            (when X != nullptr)
@@ -275,13 +302,13 @@ void TransferOp::apply(int side, const cplx* in, cplx* out) {
 }
 
 LeadingEig leading_eigenpair(Ctx& ctx, const std::function<void(const cplx*, cplx*)>& mv, int64_t n, cplx* x,
-                             int m, int max_restarts, double tol) {
+                             int m, int max_restarts, double tol, const std::function<void(cplx*)>& project) {
   hipStream_t s = ctx.stream;
   CAM_REQUIRE(n >= 1, "eigensolver: empty vector space");
   m = static_cast<int>(std::min<int64_t>(m, n));
   CAM_REQUIRE(m >= 1 && m <= 63, "eigensolver: Krylov dimension out of range");
   const int keep = std::max(1, m / 2);
-  DevBuf<cplx> V(static_cast<size_t>(m + 1) * n), w(n), tmp(static_cast<size_t>(keep) * n), coef(2 * 64 + 1),
+  DevBuf<cplx> V(static_cast<size_t>(m + 1) * n), w(n), tmp(static_cast<size_t>(keep + 1) * n), coef(2 * 64 + 1),
       partial(static_cast<size_t>(multi_dot_blocks(n)) * 64), zdev(static_cast<size_t>(m) * m);
   auto col = [&](int j) { return V.get() + static_cast<int64_t>(j) * n; };
   auto dot_self = [&](const cplx* v) {
@@ -308,9 +335,12 @@ LeadingEig leading_eigenpair(Ctx& ctx, const std::function<void(const cplx*, cpl
       ++out.matvecs;
       cplx* c1 = coef.get();
       cplx* c2 = coef.get() + 64;
+      if (project) project(w.get());
       launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c1, s);
+      if (project) hipLaunchKernelGGL(real_part_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<double2*>(c1), j + 1);
       launch_multi_axpy(V.get(), n, j + 1, c1, w.get(), n, s);
       launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c2, s);
+      if (project) hipLaunchKernelGGL(real_part_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<double2*>(c2), j + 1);
       launch_multi_axpy(V.get(), n, j + 1, c2, w.get(), n, s);
       launch_multi_dot(w.get(), n, 1, w.get(), n, partial.get(), coef.get() + 128, s);
       HIP_CHECK(hipMemcpyAsync(hc.data(), coef.get(), 129 * sizeof(cplx), hipMemcpyDeviceToHost, s));
@@ -335,7 +365,9 @@ LeadingEig leading_eigenpair(Ctx& ctx, const std::function<void(const cplx*, cpl
     for (int j = 0; j < mm; ++j)
       for (int i = 0; i < mm; ++i) T[i + static_cast<size_t>(j) * mm] = G[i + static_cast<size_t>(j) * m];
     CAM_REQUIRE(complex_schur(mm, T, Z), "eigensolver: Schur decomposition of the projected matrix failed");
-    schur_order_by_magnitude(mm, T, Z, std::min(keep, mm));
+    // one position past the kept ones is ordered too, so that a conjugate
+    // pair cut by the restart is recognised
+    schur_order_by_magnitude(mm, T, Z, std::min(keep + 1, mm));
     const zcplx theta = T[0];
     const double res = beta * std::abs(Z[mm - 1]);
     out.value = to_c(theta);
@@ -353,26 +385,167 @@ LeadingEig leading_eigenpair(Ctx& ctx, const std::function<void(const cplx*, cpl
       out.converged = conv;
       return out;
     }
-    // thick restart: V[:, :keep] = V[:, :m] Z[:, :keep], V[:, keep] = V[:, m]
-    std::vector<cplx> zk(static_cast<size_t>(mm) * keep);
-    for (int j = 0; j < keep; ++j)
-      for (int i = 0; i < mm; ++i) zk[i + static_cast<size_t>(j) * mm] = to_c(Z[i + static_cast<size_t>(j) * mm]);
+    // thick restart: V[:, :kk] = V[:, :m] Q, V[:, kk] = V[:, m], with Q an
+    // orthonormal basis of the invariant subspace of the kk leading Schur
+    // vectors; G becomes Q^H G Q with the coupling row beta e_m^T Q
+    int kk = keep;
+    bool explicit_restart = false;
+    std::vector<zcplx> Q(static_cast<size_t>(mm) * (kk + 1), 0.0);
+    if (!project) {
+      for (int j = 0; j < kk; ++j)
+        for (int i = 0; i < mm; ++i) Q[i + static_cast<size_t>(j) * mm] = Z[i + static_cast<size_t>(j) * mm];
+    } else {
+      // real basis: keep conjugate pairs together, then pivoted Gram-Schmidt
+      // on the real and imaginary parts of the kept Schur vectors
+      const zcplx a = T[(kk - 1) + static_cast<size_t>(kk - 1) * mm];
+      if (kk < mm - 1 && std::fabs(a.imag()) > 1e-12 * std::abs(a)) {
+        const zcplx b = T[kk + static_cast<size_t>(kk) * mm];
+        if (std::abs(b - std::conj(a)) <= 1e-8 * std::abs(a)) ++kk;
+      }
+      std::vector<std::vector<double>> cols;
+      for (int j = 0; j < kk; ++j) {
+        std::vector<double> re(mm), im(mm);
+        for (int i = 0; i < mm; ++i) {
+          re[i] = Z[i + static_cast<size_t>(j) * mm].real();
+          im[i] = Z[i + static_cast<size_t>(j) * mm].imag();
+        }
+        cols.push_back(std::move(re));
+        cols.push_back(std::move(im));
+      }
+      for (int r = 0; r < kk; ++r) {
+        size_t best = 0;
+        double bn = -1.0;
+        for (size_t c = 0; c < cols.size(); ++c) {
+          double nn2 = 0.0;
+          for (double v : cols[c]) nn2 += v * v;
+          if (nn2 > bn) {
+            bn = nn2;
+            best = c;
+          }
+        }
+        CAM_REQUIRE(bn > 0.0, "eigensolver: degenerate restart basis");
+        std::vector<double> q = cols[best];
+        cols.erase(cols.begin() + static_cast<long>(best));
+        for (int pass = 0; pass < 2; ++pass)
+          for (int t = 0; t < r; ++t) {
+            double d = 0.0;
+            for (int i = 0; i < mm; ++i) d += Q[i + static_cast<size_t>(t) * mm].real() * q[i];
+            for (int i = 0; i < mm; ++i) q[i] -= d * Q[i + static_cast<size_t>(t) * mm].real();
+          }
+        double nq = 0.0;
+        for (double v : q) nq += v * v;
+        nq = std::sqrt(nq);
+        CAM_REQUIRE(nq > 0.0, "eigensolver: degenerate restart basis");
+        for (int i = 0; i < mm; ++i) Q[i + static_cast<size_t>(r) * mm] = q[i] / nq;
+        for (auto& c : cols) {
+          double d = 0.0;
+          for (int i = 0; i < mm; ++i) d += Q[i + static_cast<size_t>(r) * mm].real() * c[i];
+          for (int i = 0; i < mm; ++i) c[i] -= d * Q[i + static_cast<size_t>(r) * mm].real();
+        }
+      }
+    }
+    // Gk = Q^H G Q (G still holds the projected matrix of this cycle)
+    std::vector<zcplx> GQ(static_cast<size_t>(mm) * kk, 0.0), Gk(static_cast<size_t>(kk) * kk, 0.0);
+    for (int j = 0; j < kk; ++j)
+      for (int l = 0; l < mm; ++l) {
+        const zcplx q = Q[l + static_cast<size_t>(j) * mm];
+        if (q == 0.0) continue;
+        for (int i = 0; i < mm; ++i) GQ[i + static_cast<size_t>(j) * mm] += G[i + static_cast<size_t>(l) * m] * q;
+      }
+    for (int j = 0; j < kk; ++j)
+      for (int i = 0; i < kk; ++i) {
+        zcplx acc = 0.0;
+        for (int l = 0; l < mm; ++l)
+          acc += std::conj(Q[l + static_cast<size_t>(i) * mm]) * GQ[l + static_cast<size_t>(j) * mm];
+        Gk[i + static_cast<size_t>(j) * kk] = acc;
+      }
+    if (project) {
+      // the real basis must span an invariant subspace (G Q = Q Gk); when
+      // it does not (a conjugate pair the ordering could not keep together),
+      // restart from the real dominant Ritz vector alone
+      double rn = 0.0, gn = 0.0;
+      for (const zcplx& g : G) gn += std::norm(g);
+      for (int j = 0; j < kk; ++j)
+        for (int i = 0; i < mm; ++i) {
+          zcplx qg = 0.0;
+          for (int l = 0; l < kk; ++l) qg += Q[i + static_cast<size_t>(l) * mm] * Gk[l + static_cast<size_t>(j) * kk];
+          rn += std::norm(GQ[i + static_cast<size_t>(j) * mm] - qg);
+        }
+      if (rn > 1e-20 * gn) {
+        explicit_restart = true;
+        kk = 1;
+        size_t imax = 0;
+        for (int i = 1; i < mm; ++i)
+          if (std::abs(Z[i]) > std::abs(Z[imax])) imax = static_cast<size_t>(i);
+        const zcplx ph = std::abs(Z[imax]) > 0.0 ? std::conj(Z[imax]) / std::abs(Z[imax]) : zcplx(1.0);
+        double nq = 0.0;
+        for (int i = 0; i < mm; ++i) nq += std::norm((ph * Z[i]).real());
+        nq = std::sqrt(nq);
+        CAM_REQUIRE(nq > 0.0, "eigensolver: degenerate restart vector");
+        for (int i = 0; i < mm; ++i) Q[i] = (ph * Z[i]).real() / nq;
+        zcplx acc = 0.0;
+        for (int l = 0; l < mm; ++l) {
+          zcplx gq = 0.0;
+          for (int i = 0; i < mm; ++i) gq += G[l + static_cast<size_t>(i) * m] * Q[i];
+          acc += std::conj(Q[l]) * gq;
+        }
+        Gk.assign(1, acc);
+      }
+    }
+    std::vector<cplx> zk(static_cast<size_t>(mm) * kk);
+    for (int j = 0; j < kk; ++j)
+      for (int i = 0; i < mm; ++i) zk[i + static_cast<size_t>(j) * mm] = to_c(Q[i + static_cast<size_t>(j) * mm]);
     zdev.upload(zk.data(), zk.size(), s);
-    for (int j = 0; j < keep; ++j)
+    for (int j = 0; j < kk; ++j)
       launch_combine(V.get(), n, mm, zdev.get() + static_cast<int64_t>(j) * mm, tmp.get() + static_cast<int64_t>(j) * n,
                      n, s);
-    launch_copy(tmp.get(), V.get(), static_cast<int64_t>(keep) * n, s);
-    launch_copy(col(m), col(keep), n, s);
+    launch_copy(tmp.get(), V.get(), static_cast<int64_t>(kk) * n, s);
     std::fill(G.begin(), G.end(), 0.0);
-    for (int j = 0; j < keep; ++j) {
-      for (int i = 0; i <= j; ++i) G[i + static_cast<size_t>(j) * m] = T[i + static_cast<size_t>(j) * mm];
-      G[keep + static_cast<size_t>(j) * m] = beta * Z[mm - 1 + static_cast<size_t>(j) * mm];
+    if (explicit_restart) {
+      k = 0;   // a fresh Arnoldi cycle from the (unit) dominant Ritz vector
+      continue;
     }
-    k = keep;
+    launch_copy(col(m), col(kk), n, s);
+    for (int j = 0; j < kk; ++j) {
+      for (int i = 0; i < kk; ++i) G[i + static_cast<size_t>(j) * m] = Gk[i + static_cast<size_t>(j) * kk];
+      G[kk + static_cast<size_t>(j) * m] = beta * Q[mm - 1 + static_cast<size_t>(j) * mm];
+    }
+    if (project)
+      for (auto& g : G) g = g.real();
+    k = kk;
   }
 }
 
-LeadingEig transfer_leading(Ctx& ctx, const std::vector<TensorPtr>& cell, int side, double tol, TensorPtr* M) {
+void transfer_apply(Ctx& ctx, const std::vector<TensorPtr>& cell, int side, const cplx* in, cplx* out) {
+  CAM_REQUIRE(side == 0 || side == 1, "transfer matrix: side must be 0 (left) or 1 (right)");
+  hipStream_t s = ctx.stream;
+  TransferOp T(ctx, cell);
+  const QIndex& q = *T.boundary;
+  const TransferOp::Bond& b = side == 0 ? T.bonds.front() : T.bonds.back();
+  // allowed (r, c) in the reference's loop order (c outer, r inner) -> block layout
+  std::vector<int64_t> perm;
+  perm.reserve(static_cast<size_t>(b.total));
+  for (int64_t c = 0; c < q.size; ++c)
+    for (int64_t r = 0; r < q.size; ++r) {
+      if (q.charges[r] != q.charges[c]) continue;
+      const int sct = q.pos_sec[c];
+      perm.push_back(b.off[sct] + q.pos_off[r] + b.n[sct] * q.pos_off[c]);
+    }
+  const int64_t n = b.total;
+  CAM_REQUIRE(static_cast<int64_t>(perm.size()) == n, "transfer matrix: allowed entries do not fill the blocks");
+  DevBuf<int64_t> dperm(std::max<int64_t>(n, 1));
+  dperm.upload(perm.data(), perm.size(), s);
+  DevBuf<cplx> hin(n), x(n), y(n);
+  hin.upload(in, n, s);
+  launch_scatter_map(hin.get(), x.get(), dperm.get(), n, s);
+  T.apply(side, x.get(), y.get());
+  launch_gather_map(y.get(), hin.get(), dperm.get(), n, s);
+  HIP_CHECK(hipMemcpyAsync(out, hin.get(), n * sizeof(cplx), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+LeadingEig transfer_leading(Ctx& ctx, const std::vector<TensorPtr>& cell, int side, double tol, TensorPtr* M,
+                            int ncv) {
   CAM_REQUIRE(side == 0 || side == 1, "transfer matrix: side must be 0 (left) or 1 (right)");
   hipStream_t s = ctx.stream;
   TransferOp T(ctx, cell);
@@ -384,8 +557,19 @@ LeadingEig transfer_leading(Ctx& ctx, const std::vector<TensorPtr>& cell, int si
   hipLaunchKernelGGL(diag_set_kernel, dim3(std::max(tab.nsec, 1)), dim3(256), 0, s,
                      reinterpret_cast<double2*>(x.get()), tab.off.get(), tab.n.get(), tab.nsec);
   HIP_CHECK(hipGetLastError());
+  int64_t nmax = 1;
+  for (int64_t v : b.n) nmax = std::max(nmax, v * v);
+  // The maps send Hermitian matrices to Hermitian matrices and the fixed
+  // point is Hermitian: the iteration runs on that real subspace (Hermitian
+  // part of every product, real inner products), so the projected matrix is
+  // real and a simple dominant eigenvalue comes out real to rounding.
+  auto project = [&](cplx* v) {
+    hipLaunchKernelGGL(hermitian_part_kernel, dim3(grid_of(nmax), std::max(tab.nsec, 1)), dim3(256), 0, s,
+                       reinterpret_cast<double2*>(v), tab.off.get(), tab.n.get());
+  };
   LeadingEig e = leading_eigenpair(
-      ctx, [&](const cplx* in, cplx* out) { T.apply(side, in, out); }, n, x.get(), kNcv, kMaxRestarts, tol);
+      ctx, [&](const cplx* in, cplx* out) { T.apply(side, in, out); }, n, x.get(), ncv > 0 ? ncv : kNcv, kMaxRestarts,
+      tol, project);
   if (M) {
     DevBuf<cplx> tr(1);
     hipLaunchKernelGGL(diag_sum_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<const double2*>(x.get()),
@@ -399,8 +583,6 @@ LeadingEig transfer_leading(Ctx& ctx, const std::vector<TensorPtr>& cell, int si
     const QIndex& q = side == 0 ? cell.front()->idx[1] : cell.back()->idx[2];
     *M = square_tensor(cell.front()->rules, q);
     CAM_REQUIRE((*M)->total == n, "transfer matrix: fixed-point layout mismatch");
-    int64_t nmax = 1;
-    for (int64_t v : b.n) nmax = std::max(nmax, v * v);
     hipLaunchKernelGGL(hermitian_complete_kernel, dim3(grid_of(nmax), std::max(tab.nsec, 1)), dim3(256), 0, s,
                        reinterpret_cast<const double2*>(x.get()), reinterpret_cast<double2*>((*M)->data.get()),
                        tab.off.get(), tab.n.get(), pr, pi);

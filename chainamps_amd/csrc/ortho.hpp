@@ -45,7 +45,10 @@ struct TransferOp {
 // vectors of largest magnitude), Arnoldi vectors orthogonalised by two
 // classical Gram-Schmidt passes.  x holds the start vector and receives the
 // unit eigenvector.  Converged when the Ritz residual |beta e_m^T z| is
-// <= tol |theta| (ARPACK's criterion with nev = 1, ncv = m).
+// <= tol |theta| (ARPACK's criterion with nev = 1, ncv = m).  With
+// `project`, the iteration runs on a real subspace invariant under the map:
+// project(v) maps a product onto it and the Gram-Schmidt coefficients are
+// taken real (the start vector must lie in the subspace).
 struct LeadingEig {
   cplx value{0.0, 0.0};
   int matvecs = 0, restarts = 0;
@@ -53,13 +56,20 @@ struct LeadingEig {
   bool converged = false;
 };
 LeadingEig leading_eigenpair(Ctx& ctx, const std::function<void(const cplx*, cplx*)>& mv, int64_t n, cplx* x,
-                             int m, int max_restarts, double tol);
+                             int m, int max_restarts, double tol,
+                             const std::function<void(cplx*)>& project = nullptr);
 
 // LeftED / RightED (measurement.cpp:451-580) with NUMEVALS = 1 and
 // LARGESTMAGNITUDE: the leading eigenvalue, and (when M != nullptr) the
 // Hermitian completion v + v^H of its eigenvector rotated to a real positive
 // trace, as a tensor with indices (inward q, outward q), q the boundary.
-LeadingEig transfer_leading(Ctx& ctx, const std::vector<TensorPtr>& cell, int side, double tol, TensorPtr* M);
+// ncv <= 0 takes ARPACK's 20.
+LeadingEig transfer_leading(Ctx& ctx, const std::vector<TensorPtr>& cell, int side, double tol, TensorPtr* M,
+                            int ncv = 0);
+// One application of the left (0) or right (1) map to a host vector of the
+// allowed entries in the reference's order (LeftComponentsMultiply /
+// RightComponentsMultiply: c outer, r inner, positions r + n c).
+void transfer_apply(Ctx& ctx, const std::vector<TensorPtr>& cell, int side, const cplx* in, cplx* out);
 
 struct OrthoInfo {
   LeadingEig left, right;

@@ -298,14 +298,20 @@ int cam_unitcell_destroy(cam_unitcell* u);
  * cam_idmrg_unitcell: the same from an iDMRG object's central decomposition
  * and previous lambda (iDMRG_DRV.cpp:70, getCentralDecomposition /
  * getPreviousLambda).
+ * cam_transfer_apply: LeftComponentsMultiply (side 0) / RightComponentsMultiply
+ * (side 1) (measurement.cpp:582-629) of the cell's transfer map on host
+ * vectors in and out of the allowed entries in the reference's order
+ * (allowed_indices, measurement.cpp:431-442: pairs (r, c) of equal charges,
+ * c outer, r inner), complex as (re, im) pairs.
  * cam_transfer_leading: TransferMatrixComponents(cell, 1, Identity).LeftED
  * (side 0) or .RightED (side 1) with NUMEVALS = 1 and LARGESTMAGNITUDE
  * (measurement.cpp:405-629): the leading eigenvalue value[2] (re, im) of the
- * cell's transfer map, by a device Krylov-Schur iteration (Krylov dimension
- * 20 as ARPACK's ncv, at most 50 restarts, Ritz residual <= tol |value|, tol
- * <= 0 for 1e-14); info[4] = {matvecs, restarts, relative residual,
- * converged}.  The two boundary indices of the cell must carry the same
- * charges (cells of a zero-target iDMRG and of iTEBD do).
+ * cell's transfer map, by a device Krylov-Schur iteration on the Hermitian
+ * matrices of the boundary space (Krylov dimension ncv, <= 0 for ARPACK's
+ * 20; at most 50 restarts; Ritz residual <= tol |value|, tol <= 0 for
+ * 1e-14); info[4] = {matvecs, restarts, relative residual, converged}.  The
+ * two boundary indices of the cell must carry the same charges (cells of a
+ * zero-target iDMRG and of iTEBD do).
  * cam_orthogonalise: Orthogonalise(const UnitCell&) (measurement.cpp:
  * 180-246) of a two-vertex cell: a new cell [U^H X A0 / sqrt(l), A1 X^-1 U]
  * with lambdas [new boundary values (unit square sum), the input's second
@@ -324,7 +330,8 @@ int cam_unitcell_from_decomposition(cam_ctx* ctx, const cam_tensor* U, const dou
                                     const cam_tensor* Vd, const double* previous, int64_t nprevious,
                                     cam_unitcell** out);
 int cam_idmrg_unitcell(cam_idmrg* g, cam_unitcell** out);
-int cam_transfer_leading(const cam_unitcell* u, int side, double tol, double* value, double* info);
+int cam_transfer_apply(const cam_unitcell* u, int side, const double* in, double* out);
+int cam_transfer_leading(const cam_unitcell* u, int side, double tol, int ncv, double* value, double* info);
 int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info);
 int cam_itwovertex_energy(const cam_tensor* W, const cam_unitcell* u, double* energy, double* terms);
 

@@ -170,3 +170,38 @@ def test_boundary_charge_mismatch_is_an_error(ctx):
     cell = _cell(ctx, [mats[0], bad], lams)
     with pytest.raises(cam.CamError, match="boundary indices"):
         cell.transfer_leading(0)
+
+
+def test_transfer_apply_at_size(ctx):
+    """LeftComponentsMultiply / RightComponentsMultiply on the allowed
+    entries in the reference's order, with blocks of 60-101 states (several
+    MFMA tiles per block and many K chunks per product), against the oracle's
+    dense maps (1e-12 relative)."""
+    print("case ortho transfer apply at size", flush=True)
+    rules, mats, lams = _random_cell({-1: 60, 0: 100, 1: 60}, seed=12)
+    cell = _cell(ctx, mats, lams)
+    dense = [m.to_dense_tensor() for m in mats]
+    q = mats[0].indices[1].charges
+    n = q.shape[0]
+    flat, (r, c) = ortho.allowed_entries(q)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(len(flat)) + 1j * rng.standard_normal(len(flat))
+    V = np.zeros((n, n), complex)
+    V[r, c] = x
+    for side, app in ((0, ortho.left_apply), (1, ortho.right_apply)):
+        want = app(dense, V)[r, c]
+        got = cell.transfer_apply(side, x)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), side
+
+
+def test_transfer_leading_with_restarts(ctx):
+    """A Krylov dimension of 4 forces thick restarts on the Hermitian
+    subspace; the leading eigenvalue still matches ARPACK's (1e-12)."""
+    print("case ortho transfer leading restarts", flush=True)
+    rules, mats, lams = _random_cell({-1: 7, 0: 10, 1: 7, 2: 3}, seed=6)
+    cell = _cell(ctx, mats, lams)
+    for side in (0, 1):
+        want, _, _ = ortho.transfer_leading(mats, side)
+        got, info = cell.transfer_leading(side, ncv=4)
+        assert info["converged"] and info["restarts"] > 0, info
+        assert abs(got - want) <= 1e-12 * abs(want), (side, got, want, info)
