@@ -533,7 +533,7 @@ FTEBD_FIELDS = ("step", "time", "truncation", "entropy", "overlap_re", "overlap_
 
 
 class ITEBD:
-    """Device iTEBD (TrotterDecomposition + iTEBD::evolve, orders 1 and 2)."""
+    """Device iTEBD (TrotterDecomposition + iTEBD::evolve, orders 1, 2 and 4)."""
 
     def __init__(self, W: Tensor, A0: Tensor, A1: Tensor, lambda0, lambda1, dt, order=2):
         self.ctx = W.ctx
@@ -691,6 +691,7 @@ _SIGS_ORTHO = {
     "cam_transfer_apply": [_p, C.c_int, _dp, _dp],
     "cam_transfer_leading": [_p, C.c_int, C.c_double, C.c_int, _dp, _dp],
     "cam_orthogonalise": [_p, C.POINTER(_p), _dp],
+    "cam_orthogonalise_inversion_symmetric": [_p, C.POINTER(_p), _dp],
     "cam_itwovertex_energy": [_p, _p, _dp, _dp],
 }
 _SIGS.update(_SIGS_ORTHO)
@@ -765,11 +766,13 @@ class UnitCell:
         return complex(val[0], val[1]), dict(matvecs=int(info[0]), restarts=int(info[1]), residual=info[2],
                                              converged=bool(info[3]))
 
-    def orthogonalise(self):
-        """Orthogonalise (measurement.cpp:180-246): (new cell, info)."""
+    def orthogonalise(self, inversion_symmetric=False):
+        """Orthogonalise (measurement.cpp:180-246), or
+        OrthogonaliseInversionSymmetric (131-178): (new cell, info)."""
         h = _p()
         info = np.zeros(len(ORTHO_FIELDS))
-        _check(_LIB.cam_orthogonalise(self.h, C.byref(h), _ptr(info, C.c_double)))
+        f = _LIB.cam_orthogonalise_inversion_symmetric if inversion_symmetric else _LIB.cam_orthogonalise
+        _check(f(self.h, C.byref(h), _ptr(info, C.c_double)))
         return UnitCell(self.ctx, h), dict(zip(ORTHO_FIELDS, info.tolist()))
 
     def energy(self, W: Tensor):

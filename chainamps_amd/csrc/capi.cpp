@@ -807,7 +807,7 @@ int cam_transfer_leading(const cam_unitcell* u, int side, double tol, int ncv, d
   CAM_API_END
 }
 
-int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info) {
+static int orthogonalise_cell(const cam_unitcell* u, cam_unitcell** out, double* info, bool inversion_symmetric) {
   CAM_API_BEGIN
   CAM_REQUIRE(u && out, "null argument");
   CAM_REQUIRE(u->matrices.size() == 2, "Orthogonalise: two-vertex unit cells only");
@@ -815,7 +815,11 @@ int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info) {
   TensorPtr A0n, A1n;
   std::vector<double> l0;
   cam::OrthoInfo oi;
-  cam::orthogonalise(u->owner->ctx, u->matrices[0], u->matrices[1], A0n, A1n, l0, oi);
+  if (inversion_symmetric)
+    cam::orthogonalise_inversion_symmetric(u->owner->ctx, u->matrices[0], u->matrices[1], u->lambdas[0], A0n, A1n,
+                                           l0, oi);
+  else
+    cam::orthogonalise(u->owner->ctx, u->matrices[0], u->matrices[1], A0n, A1n, l0, oi);
   cam_unitcell* r = new_cell(u->owner, {A0n, A1n}, {l0, u->lambdas[1]});
   r->basis = u->basis;
   *out = r;
@@ -828,6 +832,14 @@ int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info) {
     std::copy(v, v + 12, info);
   }
   CAM_API_END
+}
+
+int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info) {
+  return orthogonalise_cell(u, out, info, false);
+}
+
+int cam_orthogonalise_inversion_symmetric(const cam_unitcell* u, cam_unitcell** out, double* info) {
+  return orthogonalise_cell(u, out, info, true);
 }
 
 int cam_itwovertex_energy(const cam_tensor* W, const cam_unitcell* u, double* energy, double* terms) {

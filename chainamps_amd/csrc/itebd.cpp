@@ -1,38 +1,35 @@
 // Device-resident iTEBD.
 //
-// TrotterDecomposition (TEBD_routines.cpp:14-51): order 1 uses
-// {G_odd(tau), G_even(tau)}, order 2 {G_odd(tau/2), G_even(tau), G_odd(tau)};
-// order 4 canonises the unit cell between pieces
-// (OrthogonaliseInversionSymmetric), which is not on this path and is
-// rejected.  iTEBD::evolve (TEBD_routines.cpp:120-185) without measurement
-// steps: order 1 applies ops 0, 1 per step; order 2 applies op 0, then
-// (1, 2) for all but the last step, then 1, 0 and swaps the cell back.
+// TrotterDecomposition (TEBD_routines.cpp:14-85): order 1 uses
+// {G_odd(tau), G_even(tau)}, order 2 {G_odd(tau/2), G_even(tau), G_odd(tau)},
+// order 4 the 11 pieces of Suzuki's product.  iTEBD::evolve
+// (TEBD_routines.cpp:120-224) without measurement steps: order 1 applies
+// ops 0, 1 per step; order 2 applies op 0, then (1, 2) for all but the last
+// step, then 1, 0 and swaps the cell back; order 4 applies piece 0 once,
+// then per step pieces 1..9 with OrthogonaliseInversionSymmetric after each
+// even-bond piece (the odd positions), and the odd full step 2 that joins the
+// step's last half step to the next step's first.
 // Each gate is iTEBD::apply_and_decompose (TEBD_routines.cpp:233-266),
 // which leaves the cell swapped: A = {A1', U}, lambda = {new, old lambda0}.
 #include <chrono>
 #include <cmath>
 
 #include "itebd.hpp"
+#include "ftebd.hpp"
+#include "ortho.hpp"
 
 namespace cam {
 
 ITEBD::ITEBD(Ctx& c, const BSTensor& W, TensorPtr A0, TensorPtr A1, std::vector<double> l0,
              std::vector<double> l1, double time_step, int trotter_order)
     : ctx(&c), order(trotter_order), dt(time_step) {
-  CAM_REQUIRE(order == 1 || order == 2,
-              "iTEBD: Trotter orders 1 and 2 (order 4 canonises the cell between pieces)");
+  CAM_REQUIRE(order == 1 || order == 2 || order == 4, "Only 1st, 2nd and 4th order decompositions are supported");
   CAM_REQUIRE(A0 && A1 && A0->nidx() == 3 && A1->nidx() == 3, "iTEBD: the unit cell needs two 3-index tensors");
   CAM_REQUIRE(!l0.empty() && !l1.empty(), "iTEBD: empty bond weights");
-  TensorPtr Hodd = bond_hamiltonian(c, W, true);
-  TensorPtr Heven = bond_hamiltonian(c, W, false);
-  if (order == 1) {
-    ops.push_back(bond_evolution_operator(c, *Hodd, dt));
-    ops.push_back(bond_evolution_operator(c, *Heven, dt));
-  } else {
-    ops.push_back(bond_evolution_operator(c, *Hodd, 0.5 * dt));
-    ops.push_back(bond_evolution_operator(c, *Heven, dt));
-    ops.push_back(bond_evolution_operator(c, *Hodd, dt));
-  }
+  std::vector<TensorPtr> bond_ops;
+  std::vector<int> seq;
+  trotter_operators(c, W, dt, order, bond_ops, seq);
+  for (int k : seq) ops.push_back(bond_ops.at(k));   // OrderedOperatorPtrs
   A[0] = std::move(A0);
   A[1] = std::move(A1);
   lam[0] = std::move(l0);
@@ -52,6 +49,16 @@ double ITEBD::gate(int op, int64_t D, double minS) {
   return tr;
 }
 
+void ITEBD::canonise() {
+  TensorPtr n0, n1;
+  std::vector<double> nl;
+  OrthoInfo info;
+  orthogonalise_inversion_symmetric(*ctx, A[0], A[1], lam[0], n0, n1, nl, info);
+  A[0] = n0;
+  A[1] = n1;
+  lam[0] = std::move(nl);
+}
+
 ITEBDRecord ITEBD::evolve(int64_t nsteps, int64_t D, double minS) {
   CAM_REQUIRE(nsteps >= 0, "iTEBD: negative step count");
   const auto t0 = std::chrono::steady_clock::now();
@@ -60,6 +67,15 @@ ITEBDRecord ITEBD::evolve(int64_t nsteps, int64_t D, double minS) {
     for (int64_t s = 0; s < nsteps; ++s) {
       gate(0, D, minS);
       gate(1, D, minS);
+    }
+  } else if (order == 4) {
+    gate(0, D, minS);
+    for (int64_t s = 0; s < nsteps; ++s) {
+      for (int i = 1; i < 10; ++i) {
+        gate(i, D, minS);
+        if (i % 2) canonise();
+      }
+      gate(2, D, minS);
     }
   } else if (nsteps > 0) {
     gate(0, D, minS);

@@ -16,14 +16,14 @@ struct ITEBDRecord {
   double gate_flops;    // algorithmic flops of the gate contractions of this call
 };
 
-// TrotterDecomposition (TEBD_routines.cpp:14-85) for orders 1 and 2 and the
-// evolve loop (TEBD_routines.cpp:120-185) without measurement steps; the
+// TrotterDecomposition (TEBD_routines.cpp:14-85) for orders 1, 2 and 4 and
+// the evolve loop (TEBD_routines.cpp:120-224) without measurement steps; the
 // unit cell {A0, A1, lambda0, lambda1} stays on the device.
 struct ITEBD {
   Ctx* ctx;
   int order;
   double dt;
-  std::vector<TensorPtr> ops;   // BondOperators
+  std::vector<TensorPtr> ops;   // OrderedOperatorPtrs
   TensorPtr A[2];
   std::vector<double> lam[2];
   int64_t steps = 0;
@@ -33,6 +33,8 @@ struct ITEBD {
         double time_step, int trotter_order);
   // one apply_and_decompose with the unit-cell swap
   double gate(int op, int64_t D, double minS);
+  // OrthogonaliseInversionSymmetric of the cell (order 4, between pieces)
+  void canonise();
   ITEBDRecord evolve(int64_t nsteps, int64_t D, double minS);
 };
 

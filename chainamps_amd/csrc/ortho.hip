@@ -744,42 +744,75 @@ TensorPtr unit_cell_second(Ctx& ctx, const BSTensor& Vd, const std::vector<doubl
   return A1;
 }
 
-void orthogonalise(Ctx& ctx, const TensorPtr& A0, const TensorPtr& A1, TensorPtr& A0n, TensorPtr& A1n,
-                   std::vector<double>& lambda0n, OrthoInfo& info) {
-  const auto t0 = std::chrono::steady_clock::now();
-  hipStream_t s = ctx.stream;
-  const std::vector<TensorPtr> cell{A0, A1};
-  const double tol = 1e-14;
-  TensorPtr ML, MR, X, Xinv, Y;
-  auto check = [](const LeadingEig& e, const char* side) {
-    if (!e.converged && !(e.residual <= 1e-10))
-      CAM_THROW(std::string("Orthogonalise: the ") + side + " transfer-matrix eigensolve did not converge");
-  };
-  info.left = transfer_leading(ctx, cell, 0, tol, &ML);
-  check(info.left, "left");
-  if (std::fabs(info.left.value.im) >= kImagTol * std::fabs(info.left.value.re))
-    CAM_THROW("Orthogonalise: the leading left eigenvalue has a non negligible imaginary part");
-  info.kept_left = sqrt_dr(ctx, *ML, &X, &Xinv, nullptr);
-  ML.reset();
-  info.right = transfer_leading(ctx, cell, 1, tol, &MR);
-  check(info.right, "right");
-  if (std::fabs(info.right.value.im) >= kImagTol * std::fabs(info.right.value.re))
-    CAM_THROW("Orthogonalise: the leading right eigenvalue has a non negligible imaginary part");
-  info.kept_right = sqrt_dr(ctx, *MR, nullptr, nullptr, &Y);
-  MR.reset();
-  TensorPtr XLY = contract(ctx, *X, false, *Y, false, {{1, 0}});
+namespace {
+
+void check_leading(const LeadingEig& e, const char* side) {
+  if (!e.converged && !(e.residual <= 1e-10))
+    CAM_THROW(std::string("Orthogonalise: the ") + side + " transfer-matrix eigensolve did not converge");
+  if (std::fabs(e.value.im) >= kImagTol * std::fabs(e.value.re))
+    CAM_THROW(std::string("Orthogonalise: the leading ") + side +
+              " eigenvalue has a non negligible imaginary part");
+}
+
+// The common end of Orthogonalise and OrthogonaliseInversionSymmetric
+// (measurement.cpp:166-177, 229-245): SVD of XLY, values rescaled to unit
+// square sum, A0' = U^H X A0 / sqrt(l), A1' = A1 X^-1 U.
+void finish(Ctx& ctx, const TensorPtr& A0, const TensorPtr& A1, const TensorPtr& X, const TensorPtr& Xinv,
+            const BSTensor& XLY, double lead, TensorPtr& A0n, TensorPtr& A1n, std::vector<double>& lambda0n,
+            OrthoInfo& info) {
   TensorPtr U, Vd;
   double disc = 0.0;
-  svd_truncate(ctx, *XLY, 0, 0.0, U, Vd, lambda0n, disc);
+  svd_truncate(ctx, XLY, 0, 0.0, U, Vd, lambda0n, disc);
   double sq = 0.0;
   for (double v : lambda0n) sq += v * v;
   for (double& v : lambda0n) v *= std::sqrt(1.0 / sq);
   info.entropy = entropy(lambda0n);
-  launch_scale(cplx{1.0 / std::sqrt(info.left.value.re), 0.0}, X->data.get(), X->total, s);
+  launch_scale(cplx{1.0 / std::sqrt(lead), 0.0}, X->data.get(), X->total, ctx.stream);
   A0n = reorder(ctx, *contract(ctx, *contract(ctx, *U, true, *X, false, {{0, 0}}), false, *A0, false, {{1, 1}}),
                 false, {1, 0, 2}, 2);
   A1n = contract(ctx, *A1, false, *contract(ctx, *Xinv, false, *U, false, {{1, 0}}), false, {{2, 0}});
-  HIP_CHECK(hipStreamSynchronize(s));
+  HIP_CHECK(hipStreamSynchronize(ctx.stream));
+}
+
+}  // namespace
+
+void orthogonalise(Ctx& ctx, const TensorPtr& A0, const TensorPtr& A1, TensorPtr& A0n, TensorPtr& A1n,
+                   std::vector<double>& lambda0n, OrthoInfo& info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::vector<TensorPtr> cell{A0, A1};
+  const double tol = 1e-14;
+  TensorPtr ML, MR, X, Xinv, Y;
+  info.left = transfer_leading(ctx, cell, 0, tol, &ML);
+  check_leading(info.left, "left");
+  info.kept_left = sqrt_dr(ctx, *ML, &X, &Xinv, nullptr);
+  ML.reset();
+  info.right = transfer_leading(ctx, cell, 1, tol, &MR);
+  check_leading(info.right, "right");
+  info.kept_right = sqrt_dr(ctx, *MR, nullptr, nullptr, &Y);
+  MR.reset();
+  TensorPtr XLY = contract(ctx, *X, false, *Y, false, {{1, 0}});
+  finish(ctx, A0, A1, X, Xinv, *XLY, info.left.value.re, A0n, A1n, lambda0n, info);
+  info.seconds = seconds_since(t0);
+}
+
+void orthogonalise_inversion_symmetric(Ctx& ctx, const TensorPtr& A0, const TensorPtr& A1,
+                                       const std::vector<double>& lambda0, TensorPtr& A0n, TensorPtr& A1n,
+                                       std::vector<double>& lambda0n, OrthoInfo& info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::vector<TensorPtr> cell{A0, A1};
+  TensorPtr ML, X, Xinv, Y;
+  info.left = transfer_leading(ctx, cell, 0, 1e-14, &ML);
+  check_leading(info.left, "left");
+  info.right = info.left;
+  // the left fixed point serves on both boundaries (the reference runs
+  // SqrtDR twice on the same matrix; one decomposition gives X, X^-1, Y)
+  info.kept_left = info.kept_right = sqrt_dr(ctx, *ML, &X, &Xinv, &Y);
+  ML.reset();
+  // X . diag(lambda0) . Y, lambda0 on the outward index of the last matrix
+  TensorPtr Xl = reorder(ctx, *X, false, {0, 1}, 1);
+  scale_along_index(ctx, *Xl, 1, lambda0);
+  TensorPtr XLY = contract(ctx, *Xl, false, *Y, false, {{1, 0}});
+  finish(ctx, A0, A1, X, Xinv, *XLY, info.left.value.re, A0n, A1n, lambda0n, info);
   info.seconds = seconds_since(t0);
 }
 

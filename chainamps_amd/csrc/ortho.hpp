@@ -89,6 +89,14 @@ TensorPtr unit_cell_second(Ctx& ctx, const BSTensor& Vd, const std::vector<doubl
 void orthogonalise(Ctx& ctx, const TensorPtr& A0, const TensorPtr& A1, TensorPtr& A0n, TensorPtr& A1n,
                    std::vector<double>& lambda0n, OrthoInfo& info);
 
+// OrthogonaliseInversionSymmetric (measurement.cpp:131-178): only the left
+// fixed point is solved and serves on the right boundary too; the boundary
+// lambda lambda0 (on the last matrix's outward index) enters between the
+// factors, SVD of X . diag(lambda0) . Y.
+void orthogonalise_inversion_symmetric(Ctx& ctx, const TensorPtr& A0, const TensorPtr& A1,
+                                       const std::vector<double>& lambda0, TensorPtr& A0n, TensorPtr& A1n,
+                                       std::vector<double>& lambda0n, OrthoInfo& info);
+
 // iTwoVertexEnergy (measurement.cpp:371-397) of the orthogonalised cell
 // [A0, A1] with boundary lambda lambda0, from the Hamiltonian MPO W.
 // terms: on-vertex, bond inside the cell, bond to the next cell.

@@ -224,11 +224,14 @@ int cam_idmrg_destroy(cam_idmrg* d);
  * iTEBD (TEBD_routines.cpp:14-85 TrotterDecomposition, 120-185 evolve,
  * 233-266 apply_and_decompose) with the unit cell on the device: W the
  * model MPO, A0/A1 the left-shape cell tensors [s, a_l, a_r], lambda0/1
- * the bond weights, dt the time step, order 1 or 2 (order 4 needs the
- * cell canonisation between Trotter pieces and is rejected).  evolve runs
+ * the bond weights, dt the time step, order 1, 2 or 4.  evolve runs
  * nsteps full time steps with no measurement in between; the second-order
  * step is odd(dt/2) even(dt) odd(dt) ... even(dt) odd(dt/2) with the odd
- * full-step operator between steps.  record[7] receives {steps so far,
+ * full-step operator between steps; the fourth-order run applies piece 0
+ * once, then per step pieces 1..9 of Suzuki's product with
+ * OrthogonaliseInversionSymmetric (measurement.cpp:131-178) after each
+ * even-bond piece, then the odd full step joining two steps (as the
+ * reference, the last step's closing odd half step stays applied).  record[7] receives {steps so far,
  * time, truncation of the last gate, entropy of lambda0, chi, seconds,
  * gate-contraction flops of the call}. */
 typedef struct cam_itebd_s cam_itebd;
@@ -320,6 +323,11 @@ int cam_unitcell_destroy(cam_unitcell* u);
  * info[12] = {left value re, im, right value re, im, entropy, new bond
  * dimension, left matvecs, right matvecs, left residual, right residual,
  * kept SqrtDR vectors (left + right), seconds}.
+ * cam_orthogonalise_inversion_symmetric: OrthogonaliseInversionSymmetric
+ * (measurement.cpp:131-178): as cam_orthogonalise with only the left fixed
+ * point, reused on the right boundary, and the SVD of X . diag(lambda0) . Y
+ * (lambda0 = the input cell's first lambda); info as cam_orthogonalise (the
+ * right value and matvecs repeat the left ones).
  * cam_itwovertex_energy: iTwoVertexEnergy(H_MPO, Ortho) (measurement.cpp:
  * 371-397) of an orthogonalised two-vertex cell: energy[2] (re, im) per
  * vertex; terms[6] = (on-vertex, bond inside the cell, bond to the next
@@ -333,6 +341,7 @@ int cam_idmrg_unitcell(cam_idmrg* g, cam_unitcell** out);
 int cam_transfer_apply(const cam_unitcell* u, int side, const double* in, double* out);
 int cam_transfer_leading(const cam_unitcell* u, int side, double tol, int ncv, double* value, double* info);
 int cam_orthogonalise(const cam_unitcell* u, cam_unitcell** out, double* info);
+int cam_orthogonalise_inversion_symmetric(const cam_unitcell* u, cam_unitcell** out, double* info);
 int cam_itwovertex_energy(const cam_tensor* W, const cam_unitcell* u, double* energy, double* terms);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------

@@ -588,6 +588,13 @@ inline UnitCell Orthogonalise(const UnitCell& C) {
   return UnitCell::FromHandle(out);
 }
 
+/* OrthogonaliseInversionSymmetric (measurement.cpp:131-178). */
+inline UnitCell OrthogonaliseInversionSymmetric(const UnitCell& C) {
+  cam_unitcell* out = nullptr;
+  check(cam_orthogonalise_inversion_symmetric(C.handle().get(), &out, nullptr));
+  return UnitCell::FromHandle(out);
+}
+
 /* iTwoVertexEnergy (measurement.cpp:371-397): energy per vertex of an
  * orthogonalised two-vertex cell. */
 inline std::complex<double> iTwoVertexEnergy(const MPX_matrix& H_MPO, const UnitCell& Ortho) {

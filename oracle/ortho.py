@@ -21,6 +21,8 @@ Restates:
   * SqrtDR (MPX_matrix.cpp:829-913): block Hermitian eigensolve, eigenvalues
     with |w| <= 100 eps max|w| dropped with their vectors, mixed signs
     rejected, returns sqrt|w| and R = V^H (rows on a deduced index);
+  * OrthogonaliseInversionSymmetric (measurement.cpp:131-178): the left
+    fixed point only, reused on the right boundary, SVD of X.diag(l0).Y;
   * Orthogonalise (measurement.cpp:180-246): X = diag(sqrt w_L) R_L,
     X^-1 = R_L^H diag(1/sqrt w_L), Y = R_R^H diag(sqrt w_R), SVD of X.Y,
     singular values rescaled to unit square sum, A0' = U^H X A0 / sqrt(l_L),
@@ -218,6 +220,16 @@ def orthogonalise(mats, lambdas):
     LY = contract(reorder(RR, True, [1, 0], 1), False,
                   diagonal_mpx(rules, RR.indices[0], RR.indices[0], rootsR), False, [(1, 0)])
     XLY = contract(X, False, LY, False, [(1, 0)])
+    return _finish(mats, lambdas, X, Xinv, XLY, lamL, dict(left_value=lamL, right_value=lamR,
+                                                           kept_left=len(rootsL), kept_right=len(rootsR)))
+
+
+def _finish(mats, lambdas, X, Xinv, XLY, lamL, info):
+    """The common end of Orthogonalise and OrthogonaliseInversionSymmetric
+    (measurement.cpp:166-177, 229-245): SVD of X.(lambda).Y, values rescaled
+    to unit square sum, A0' = U^H X A0 / sqrt(l), A1' = A1 X^-1 U."""
+    A0, A1 = mats
+    rules = A0.rules
     svd = mpx_svd(XLY)
     vals = np.asarray(svd["values"], float)
     vals = vals * np.sqrt(1.0 / float(np.sum(vals * vals)))
@@ -227,9 +239,39 @@ def orthogonalise(mats, lambdas):
     A0n = reorder(contract(contract(U, True, sX, False, [(0, 0)]), False, A0, False, [(1, 1)]), False, [1, 0, 2], 2)
     A1n = contract(A1, False, contract(Xinv, False, U, False, [(1, 0)]), False, [(2, 0)])
     nz = vals[vals > 0]
-    info = dict(left_value=lamL, right_value=lamR, entropy=float(-np.sum(nz * nz * np.log(nz * nz))),
-                kept_left=len(rootsL), kept_right=len(rootsR))
+    info = dict(info, entropy=float(-np.sum(nz * nz * np.log(nz * nz))))
     return [A0n, A1n], [vals, np.asarray(lambdas[1], float).copy()], info
+
+
+def orthogonalise_inversion_symmetric(mats, lambdas):
+    """OrthogonaliseInversionSymmetric (measurement.cpp:131-178): only the
+    left fixed point is solved; under inversion symmetry it serves as the
+    right one on the outward index of the last matrix, and the boundary
+    lambda enters between the factors: SVD of X . diag(lambda0) . Y."""
+    A0, A1 = mats
+    rules = A0.rules
+    lamL, ML, _ = transfer_leading(mats, 0)
+    if abs(lamL.imag) >= IMAGTOL * abs(lamL.real):
+        raise OrthogonaliseError("Eigenvalue has non negligible imaginary part")
+    q0 = A0.indices[1].charges
+    dl = sqrt_dr(MPX(rules, [MPXIndex(True, q0), MPXIndex(False, q0)], 1, sp.csc_matrix(ML)))
+    if dl is None:
+        raise OrthogonaliseError("SqrtDR failure")
+    rootsL, RL = dl
+    X = contract(diagonal_mpx(rules, RL.indices[0], RL.indices[0], rootsL), False, RL, False, [(1, 0)])
+    Xinv = contract(reorder(RL, True, [1, 0], 1), False,
+                    diagonal_mpx(rules, RL.indices[0], RL.indices[0], rootsL, inverse=True), False, [(1, 0)])
+    q1 = A1.indices[2].charges
+    dr = sqrt_dr(MPX(rules, [MPXIndex(True, q1), MPXIndex(False, q1)], 1, sp.csc_matrix(ML)))
+    if dr is None:
+        raise OrthogonaliseError("SqrtDR failure")
+    rootsR, RR = dr
+    Y = contract(reorder(RR, True, [1, 0], 1), False,
+                 diagonal_mpx(rules, RR.indices[0], RR.indices[0], rootsR), False, [(1, 0)])
+    L0 = diagonal_mpx(rules, A1.indices[2], A1.indices[2], lambdas[0])
+    XLY = contract(contract(X, False, L0, False, [(1, 0)]), False, Y, False, [(1, 0)])
+    return _finish(mats, lambdas, X, Xinv, XLY, lamL, dict(left_value=lamL, right_value=lamL,
+                                                           kept_left=len(rootsL), kept_right=len(rootsR)))
 
 
 # ---------------------------------------------------------------------------

@@ -91,19 +91,24 @@ def product_state_cell(rules, phys, level=0):
 
 
 def trotter_operators(W: MPX, tau: float, order: int):
-    """TrotterDecomposition (TEBD_routines.cpp:14-51): the bond operators of
-    order 1 (odd tau, even tau) and order 2 (odd tau/2, even tau, odd tau)."""
+    """TrotterDecomposition (TEBD_routines.cpp:14-85): OrderedOperatorPtrs
+    of order 1 (odd tau, even tau), order 2 (odd tau/2, even tau, odd tau)
+    and order 4 (the 11 pieces of Suzuki's product)."""
     Hodd, Heven = odd_bond_hamiltonian(W), even_bond_hamiltonian(W)
     if order == 1:
         return [bond_evolution_operator(Hodd, tau), bond_evolution_operator(Heven, tau)]
     if order == 2:
         return [bond_evolution_operator(Hodd, 0.5 * tau), bond_evolution_operator(Heven, tau),
                 bond_evolution_operator(Hodd, tau)]
-    raise ValueError("order 1 or 2 (order 4 canonises with OrthogonaliseInversionSymmetric)")
+    if order == 4:
+        from .ftebd import trotter_operators as pieces
+        ops, seq = pieces(W, tau, 4)
+        return [ops[k] for k in seq]
+    raise ValueError("Only 1st, 2nd and 4th order decompositions are supported")
 
 
 def evolve(ops, order, cell, steps, D=0, minS=0.0):
-    """iTEBD::evolve (TEBD_routines.cpp:120-185) for orders 1 and 2 with no
+    """iTEBD::evolve (TEBD_routines.cpp:120-224) for orders 1, 2 and 4 with no
     measurement step inside the run.  ``cell`` = [A0, A1, lambda0, lambda1];
     returns the new cell and the truncation of the last gate.
 
@@ -134,6 +139,19 @@ def evolve(ops, order, cell, steps, D=0, minS=0.0):
             gate(ops[1])
             gate(ops[0])
             A0, A1, l0, l1 = A1, A0, l1, l0         # m_unit.swap(0,1)
+    elif order == 4:
+        # TEBD_routines.cpp:194-224 without measurement steps: the special
+        # start, pieces 1..9 re-canonised after every even-bond piece, and
+        # the odd full step that joins this step's last piece to the next
+        # step's first
+        from .ortho import orthogonalise_inversion_symmetric
+        gate(ops[0])
+        for _ in range(steps):
+            for i in range(1, 10):
+                gate(ops[i])
+                if i % 2:
+                    (A0, A1), (l0, l1), _ = orthogonalise_inversion_symmetric([A0, A1], [l0, l1])
+            gate(ops[2])
     else:
-        raise ValueError("order 1 or 2")
+        raise ValueError("order 1, 2 or 4")
     return [A0, A1, l0, l1], tr

@@ -61,9 +61,29 @@ def test_itebd_unitary_without_truncation(ctx):
     assert rec["gate_flops"] > 0
 
 
-def test_itebd_rejects_order_four(ctx):
+def test_itebd_order_four_matches_oracle(ctx):
+    """Fourth order (11 pieces per step, OrthogonaliseInversionSymmetric
+    after every even-bond piece; TEBD_routines.cpp:52-85, 194-224): the
+    cell's two lambdas follow the oracle (1e-8), and so do the values of
+    the inversion-symmetric canonisation of the final cell."""
+    print("case itebd order 4", flush=True)
     import chainamps_amd as cam
-    rules, phys, W, E, sig = models.coupled_chain_model(d=3, seed=8, Jperp=0.3)
+    from oracle import ortho
+    rules, phys, W, E, sig = models.coupled_chain_model(d=4, seed=2, Jperp=0.4)
     A0, A1, l0, l1 = tebd.product_state_cell(rules, phys, 0)
-    with pytest.raises(cam.CamError):
-        cam.ITEBD(to_lib(ctx, W), to_lib(ctx, A0), to_lib(ctx, A1), l0, l1, 0.02, 4)
+    dt, D, steps = 0.1, 24, 2
+    ops = tebd.trotter_operators(W, dt, 4)
+    cell, tr = tebd.evolve(ops, 4, [A0, A1, l0, l1], steps, D, 1e-12)
+    dev = cam.ITEBD(to_lib(ctx, W), to_lib(ctx, A0), to_lib(ctx, A1), l0, l1, dt, 4)
+    rec = dev.evolve(steps, D, 1e-12)
+    for k in (0, 1):
+        got, want = np.sort(dev.lam(k))[::-1], np.sort(cell[2 + k])[::-1]
+        assert got.shape == want.shape, k
+        assert np.allclose(got, want, rtol=1e-8, atol=1e-11), k
+    # OrthogonaliseInversionSymmetric alone, on the oracle's final cell
+    mats, lams = [cell[0], cell[1]], [cell[2], cell[3]]
+    new, nl, info = ortho.orthogonalise_inversion_symmetric(mats, lams)
+    c = cam.UnitCell.create(ctx, [to_lib(ctx, m) for m in mats], lams)
+    o, dinfo = c.orthogonalise(inversion_symmetric=True)
+    assert np.allclose(np.sort(o.lam(0)), np.sort(nl[0]), rtol=1e-8, atol=1e-11)
+    assert abs(dinfo["entropy"] - info["entropy"]) < 1e-9
