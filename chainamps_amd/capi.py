@@ -71,6 +71,7 @@ _SIGS = {
     "cam_heff_flops": [_p, _dp],
     "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
+    "cam_heff_apply_blocked32": [_p, _p, _p],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
     "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
@@ -320,6 +321,11 @@ class Heff:
             _check(_LIB.cam_heff_apply_blocked(self.h, din, dout, precision))
         else:
             _check(_LIB.cam_heff_apply(self.h, din, dout, CAM_MEM_DEVICE, precision))
+
+    def apply_device32(self, din, dout):
+        """din/dout: device pointers (ints) of n complex64 (split-complex
+        FP32) in the block layout: the mixed-precision solver's matvec."""
+        _check(_LIB.cam_heff_apply_blocked32(self.h, din, dout))
 
     def profile(self, precision=CAM_FP64, reps=5):
         """Average ms per stage (L, W, W, R) from HIP events and the

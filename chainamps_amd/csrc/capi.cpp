@@ -270,6 +270,14 @@ int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int preci
   CAM_API_END
 }
 
+int cam_heff_apply_blocked32(cam_heff* h, const float* in, float* out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(h && in && out, "null argument");
+  set_device(h->owner);
+  h->h.apply_blocked32(reinterpret_cast<const float2*>(in), reinterpret_cast<float2*>(out));
+  CAM_API_END
+}
+
 int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, double* stage_flops) {
   CAM_API_BEGIN
   set_device(h->owner);

@@ -102,9 +102,19 @@ __global__ void copy_kernel(const double2* __restrict__ x, double2* __restrict__
 constexpr int kMaxK = 64;
 constexpr int kDotBlocks = 1024;
 
-// Each workgroup reduces a contiguous slice of rows for all k columns.
-__global__ __launch_bounds__(256) void multi_dot_kernel(const double2* __restrict__ V, int64_t ldv,
-                                                        int k, const double2* __restrict__ w,
+__device__ __forceinline__ double2 ld2(const double2* p) { return *p; }
+__device__ __forceinline__ double2 ld2(const float2* p) {
+  const float2 v = *p;
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(double2* p, double2 v) { *p = v; }
+__device__ __forceinline__ void st2(float2* p, double2 v) { *p = make_float2(static_cast<float>(v.x), static_cast<float>(v.y)); }
+
+// Each workgroup reduces a contiguous slice of rows for all k columns (FP64
+// accumulation whatever the storage of V and w).
+template <typename TV, typename TW>
+__global__ __launch_bounds__(256) void multi_dot_kernel(const TV* __restrict__ V, int64_t ldv,
+                                                        int k, const TW* __restrict__ w,
                                                         int64_t n, double2* __restrict__ partial) {
   __shared__ double red_r[kMaxK][8];
   __shared__ double red_i[kMaxK][8];
@@ -117,11 +127,11 @@ __global__ __launch_bounds__(256) void multi_dot_kernel(const double2* __restric
 #pragma unroll
     for (int q = 0; q < 8; ++q) { sr[q] = 0; si[q] = 0; }
     for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const double2 b = w[i];
+      const double2 b = ld2(w + i);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         if (j0 + q < k) {
-          const double2 a = V[i + (j0 + q) * ldv];
+          const double2 a = ld2(V + i + (j0 + q) * ldv);
           sr[q] += a.x * b.x + a.y * b.y;
           si[q] += a.x * b.y - a.y * b.x;
         }
@@ -174,9 +184,11 @@ __global__ void multi_dot_final(const double2* __restrict__ partial, int nb, int
   if (threadIdx.x == 0) c[j] = make_double2(sr[0], si[0]);
 }
 
-__global__ void multi_axpy_kernel(const double2* __restrict__ V, int64_t ldv, int k,
-                                  const double2* __restrict__ c, double2* __restrict__ w, int64_t n,
-                                  int accumulate_negative) {
+// mode 1: w -= V c; mode 0: w = V c; mode 2: w += V c (FP64 arithmetic)
+template <typename TV, typename TW>
+__global__ void multi_axpy_kernel(const TV* __restrict__ V, int64_t ldv, int k,
+                                  const double2* __restrict__ c, TW* __restrict__ w, int64_t n,
+                                  int mode) {
   __shared__ double2 cc[kMaxK];
   if (threadIdx.x < k) cc[threadIdx.x] = c[threadIdx.x];
   __syncthreads();
@@ -184,18 +196,19 @@ __global__ void multi_axpy_kernel(const double2* __restrict__ V, int64_t ldv, in
        i += int64_t(gridDim.x) * blockDim.x) {
     double r = 0, im = 0;
     for (int j = 0; j < k; ++j) {
-      const double2 a = V[i + j * ldv];
+      const double2 a = ld2(V + i + j * ldv);
       const double2 b = cc[j];
       r += a.x * b.x - a.y * b.y;
       im += a.x * b.y + a.y * b.x;
     }
-    if (accumulate_negative) {
-      double2 o = w[i];
-      o.x -= r;
-      o.y -= im;
-      w[i] = o;
+    if (mode) {
+      double2 o = ld2(w + i);
+      const double sgn = mode == 1 ? -1.0 : 1.0;
+      o.x += sgn * r;
+      o.y += sgn * im;
+      st2(w + i, o);
     } else {
-      w[i] = make_double2(r, im);
+      st2(w + i, make_double2(r, im));
     }
   }
 }
@@ -319,7 +332,7 @@ void launch_multi_dot(const cplx* V, int64_t ldv, int k, const cplx* w, int64_t 
                       cplx* c, hipStream_t s) {
   CAM_REQUIRE(k >= 1 && k <= kMaxK, "multi_dot: k out of range");
   const int nb = multi_dot_blocks(n);
-  hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(256), 0, s,
+  hipLaunchKernelGGL((multi_dot_kernel<double2, double2>), dim3(nb), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(V), ldv, k,
                      reinterpret_cast<const double2*>(w), n, reinterpret_cast<double2*>(partial));
   HIP_CHECK(hipGetLastError());
@@ -331,7 +344,7 @@ void launch_multi_dot(const cplx* V, int64_t ldv, int k, const cplx* w, int64_t 
 void launch_multi_axpy(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n,
                        hipStream_t s) {
   CAM_REQUIRE(k >= 1 && k <= kMaxK, "multi_axpy: k out of range");
-  hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, s,
+  hipLaunchKernelGGL((multi_axpy_kernel<double2, double2>), dim3(grid_for(n)), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(V), ldv, k,
                      reinterpret_cast<const double2*>(c), reinterpret_cast<double2*>(w), n, 1);
   HIP_CHECK(hipGetLastError());
@@ -340,9 +353,35 @@ void launch_multi_axpy(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w
 void launch_combine(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n,
                     hipStream_t s) {
   CAM_REQUIRE(k >= 1 && k <= kMaxK, "combine: k out of range");
-  hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, s,
+  hipLaunchKernelGGL((multi_axpy_kernel<double2, double2>), dim3(grid_for(n)), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(V), ldv, k,
                      reinterpret_cast<const double2*>(c), reinterpret_cast<double2*>(w), n, 0);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_multi_dot32(const float2* V, int64_t ldv, int k, const float2* w, int64_t n, cplx* partial, cplx* c,
+                        hipStream_t s) {
+  CAM_REQUIRE(k >= 1 && k <= kMaxK, "multi_dot: k out of range");
+  const int nb = multi_dot_blocks(n);
+  hipLaunchKernelGGL((multi_dot_kernel<float2, float2>), dim3(nb), dim3(256), 0, s, V, ldv, k, w, n,
+                     reinterpret_cast<double2*>(partial));
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(multi_dot_final, dim3(k), dim3(256), 0, s, reinterpret_cast<const double2*>(partial), nb, k,
+                     reinterpret_cast<double2*>(c));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_multi_axpy32(const float2* V, int64_t ldv, int k, const cplx* c, float2* w, int64_t n, hipStream_t s) {
+  CAM_REQUIRE(k >= 1 && k <= kMaxK, "multi_axpy: k out of range");
+  hipLaunchKernelGGL((multi_axpy_kernel<float2, float2>), dim3(grid_for(n)), dim3(256), 0, s, V, ldv, k,
+                     reinterpret_cast<const double2*>(c), w, n, 1);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_accumulate32(const float2* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, hipStream_t s) {
+  CAM_REQUIRE(k >= 1 && k <= kMaxK, "accumulate: k out of range");
+  hipLaunchKernelGGL((multi_axpy_kernel<float2, double2>), dim3(grid_for(n)), dim3(256), 0, s, V, ldv, k,
+                     reinterpret_cast<const double2*>(c), reinterpret_cast<double2*>(w), n, 2);
   HIP_CHECK(hipGetLastError());
 }
 

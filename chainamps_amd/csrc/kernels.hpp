@@ -52,6 +52,12 @@ void launch_multi_axpy(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w
 void launch_combine(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n,
                     hipStream_t s);
 int multi_dot_blocks(int64_t n);
+// The same on FP32 (split-complex float2) storage, FP64 accumulation:
+// c = V^H w; w -= V c; and w (FP64) += V c.
+void launch_multi_dot32(const float2* V, int64_t ldv, int k, const float2* w, int64_t n, cplx* partial, cplx* c,
+                        hipStream_t s);
+void launch_multi_axpy32(const float2* V, int64_t ldv, int k, const cplx* c, float2* w, int64_t n, hipStream_t s);
+void launch_accumulate32(const float2* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, hipStream_t s);
 // x[i] = (a, b), a and b uniform in [-1, 1) from a counter-based splitmix64
 // stream (deterministic for a given seed)
 void launch_random_fill(cplx* x, int64_t n, uint64_t seed, hipStream_t s);

@@ -55,7 +55,8 @@ namespace {
 // breaks down when beta <= 16 eps max(1, |alpha|), alpha = Re of the summed
 // CGS2 coefficients c1[j] + c2[j].  Writes v_{j+1} = w / beta (zero after a
 // breakdown) and the decision {beta, flag} for the host.
-__global__ void lanczos_next_kernel(const double2* __restrict__ w, double2* __restrict__ next, int64_t n,
+template <typename T>
+__global__ void lanczos_next_kernel(const T* __restrict__ w, T* __restrict__ next, int64_t n,
                                     const double2* __restrict__ nrm, const double2* __restrict__ c1,
                                     const double2* __restrict__ c2, double eps16, double2* __restrict__ dec) {
   const double alpha = c1->x + c2->x;
@@ -66,8 +67,8 @@ __global__ void lanczos_next_kernel(const double2* __restrict__ w, double2* __re
   if (!next) return;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
-    const double2 v = w[i];
-    next[i] = make_double2(v.x * inv, v.y * inv);
+    const T v = w[i];
+    next[i] = T{static_cast<decltype(v.x)>(v.x * inv), static_cast<decltype(v.x)>(v.y * inv)};
   }
 }
 
@@ -78,6 +79,11 @@ struct Lanczos {
   int m;                      // Krylov dimension
   DevView<cplx> V;            // (m+1) x n, in the context's Krylov workspace
   DevView<cplx> w;            // work (the Ritz vector lives in the caller's buffer)
+  // FP32 cycles keep their Krylov vectors in FP32 (split complex), in the
+  // first half of the same workspace: columns 0..m and the work vector w32,
+  // all below the FP64 work vector w
+  float2* V32 = nullptr;
+  float2* w32 = nullptr;
   DevBuf<cplx> coef, partial;
   DevBuf<cplx> C1, C2, Nrm, Dec;   // per-step CGS2 coefficients, norms, decisions
   DevBuf<double> dD, dE, dC;
@@ -92,6 +98,8 @@ struct Lanczos {
     h.ctx->ws_krylov.ensure(static_cast<size_t>(n) * (m + 2));
     V.p = h.ctx->ws_krylov.get();
     w.p = V.p + static_cast<size_t>(n) * (m + 1);
+    V32 = reinterpret_cast<float2*>(V.p);
+    w32 = V32 + static_cast<size_t>(n) * (m + 1);
     coef.alloc(64);
     partial.alloc(static_cast<size_t>(multi_dot_blocks(n)) * 64);
     C1.alloc(64 * 64);
@@ -112,6 +120,7 @@ struct Lanczos {
   Lanczos(const Lanczos&) = delete;
   Lanczos& operator=(const Lanczos&) = delete;
   cplx* col(int j) { return V.get() + static_cast<int64_t>(j) * n; }
+  float2* col32(int j) { return V32 + static_cast<int64_t>(j) * n; }
 
   // Steps j = 1 .. m-1 of one Krylov cycle with every scalar kept on the
   // device (col(1) must hold the normalised residual of step 0): matvec,
@@ -119,6 +128,27 @@ struct Lanczos {
   // whole sequence is one hipGraph.
   void enqueue_cycle(int prec) {
     const double eps16 = 16 * std::numeric_limits<double>::epsilon();
+    if (prec == 1) {
+      // FP32 vectors (col32(0) = x, col32(1) = the normalised residual):
+      // FP32 matvecs and Gram-Schmidt on FP32 storage, FP64 sums
+      for (int j = 1; j < m; ++j) {
+        H.apply_blocked32(col32(j), w32);
+        cplx* c1 = C1.get() + j * 64;
+        cplx* c2 = C2.get() + j * 64;
+        launch_multi_dot32(V32, n, j + 1, w32, n, partial.get(), c1, s);
+        launch_multi_axpy32(V32, n, j + 1, c1, w32, n, s);
+        launch_multi_dot32(V32, n, j + 1, w32, n, partial.get(), c2, s);
+        launch_multi_axpy32(V32, n, j + 1, c2, w32, n, s);
+        launch_multi_dot32(w32, n, 1, w32, n, partial.get(), Nrm.get() + j, s);
+        const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)));
+        hipLaunchKernelGGL(lanczos_next_kernel<float2>, dim3(grid), dim3(256), 0, s, w32,
+                           j + 1 < m ? col32(j + 1) : nullptr, n, reinterpret_cast<const double2*>(Nrm.get() + j),
+                           reinterpret_cast<const double2*>(c1 + j), reinterpret_cast<const double2*>(c2 + j),
+                           eps16, reinterpret_cast<double2*>(Dec.get() + j));
+        HIP_CHECK(hipGetLastError());
+      }
+      return;
+    }
     for (int j = 1; j < m; ++j) {
       H.apply(col(j), w.get(), prec);
       cplx* c1 = C1.get() + j * 64;
@@ -132,7 +162,7 @@ struct Lanczos {
       launch_multi_axpy(V.get(), n, j + 1, c2, w.get(), n, s);
       launch_multi_dot(w.get(), n, 1, w.get(), n, partial.get(), Nrm.get() + j, s);
       const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)));
-      hipLaunchKernelGGL(lanczos_next_kernel, dim3(grid), dim3(256), 0, s,
+      hipLaunchKernelGGL(lanczos_next_kernel<double2>, dim3(grid), dim3(256), 0, s,
                          reinterpret_cast<const double2*>(w.get()),
                          j + 1 < m ? reinterpret_cast<double2*>(col(j + 1)) : nullptr, n,
                          reinterpret_cast<const double2*>(Nrm.get() + j),
@@ -146,7 +176,10 @@ struct Lanczos {
   bool run_cycle(int prec, std::vector<double>& alpha, std::vector<double>& beta, int& k) {
     if (m > 1) {
       // launch configurations are fixed before they are captured
-      if (!H.tuned(prec)) H.apply(col(1), w.get(), prec);
+      if (!H.tuned(prec)) {
+        if (prec == 1) H.apply_blocked32(col32(1), w32);
+        else H.apply(col(1), w.get(), prec);
+      }
       hipGraphExec_t& ge = cycle_exec[prec];
       if (!ge && use_graphs) {
         HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -372,11 +405,18 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
     std::vector<double> alpha{theta}, beta;
     bool invariant = resid <= 16 * eps * std::max(1.0, std::fabs(theta));
     int k = 1;
+    const bool cycle32 = use32;
     if (!invariant) {
       beta.push_back(resid);
       launch_scale(cplx{1.0 / resid, 0.0}, L.w.get(), n, s);
-      launch_copy(L.w.get(), L.col(1), n, s);
-      invariant = L.run_cycle(use32 ? 1 : 0, alpha, beta, k);
+      if (cycle32) {
+        // x and the residual in FP32 (col(0)'s FP64 copy of x is dead now)
+        launch_to_fp32(xout, L.col32(0), n, s);
+        launch_to_fp32(L.w.get(), L.col32(1), n, s);
+      } else {
+        launch_copy(L.w.get(), L.col(1), n, s);
+      }
+      invariant = L.run_cycle(cycle32 ? 1 : 0, alpha, beta, k);
     }
     std::vector<double> svec;
     beta.resize(k, 0.0);
@@ -385,7 +425,14 @@ double lanczos_lowest(Heff& H, cplx* xout, bool have_start, bool converge, doubl
     std::vector<std::complex<double>> cs(k);
     for (int j = 0; j < k; ++j) cs[j] = svec[j];
     HIP_CHECK(hipMemcpyAsync(L.coef.get(), cs.data(), k * sizeof(cplx), hipMemcpyHostToDevice, s));
-    launch_combine(L.V.get(), n, k, L.coef.get(), xout, n, s);
+    if (cycle32) {
+      // x <- s_0 x + sum_{j>=1} s_j v_j: the FP64 x keeps its precision and
+      // only the correction comes from the FP32 vectors
+      launch_scale(cplx{svec[0], 0.0}, xout, n, s);
+      if (k > 1) launch_accumulate32(L.col32(1), n, k - 1, L.coef.get() + 1, xout, n, s);
+    } else {
+      launch_combine(L.V.get(), n, k, L.coef.get(), xout, n, s);
+    }
     const double nx = L.norm(xout);
     launch_scale(cplx{1.0 / nx, 0.0}, xout, n, s);
     ++restarts;

@@ -117,6 +117,10 @@ int cam_heff_apply(cam_heff* h, const double* in, double* out, int mem_kind, int
 /* Same matvec on vectors already in the device block layout (the Lanczos
  * layout), no gather/scatter; `in`/`out` device pointers of n complex. */
 int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int precision);
+/* The FP32 matvec on FP32 split-complex vectors in the device block layout
+ * (n complex = 2n floats each), as the mixed-precision eigensolver runs it:
+ * no FP64 conversion of in or out. */
+int cam_heff_apply_blocked32(cam_heff* h, const float* in, float* out);
 /* Benchmark helper (no reference counterpart): run the matvec `reps` times
  * on internal random block-layout vectors and return, per stage
  * (L, W, W, R), the average launch duration in ms measured with HIP events
