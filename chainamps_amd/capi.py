@@ -72,6 +72,7 @@ _SIGS = {
     "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
     "cam_heff_apply_blocked32": [_p, _p, _p],
+    "cam_heff_set_projectors": [_p, C.c_int, C.POINTER(_dp), _dp],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
     "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
@@ -321,6 +322,18 @@ class Heff:
             _check(_LIB.cam_heff_apply_blocked(self.h, din, dout, precision))
         else:
             _check(_LIB.cam_heff_apply(self.h, din, dout, CAM_MEM_DEVICE, precision))
+
+    def set_projectors(self, projectors):
+        """(P, weight) pairs: P the projector tensor's entries in the
+        reference compressed order (TwoVertexComponents' ProjectorTensors)."""
+        vecs = [np.ascontiguousarray(P, dtype=np.complex128) for P, _ in projectors]
+        for v in vecs:
+            if v.shape != (self.n,):
+                raise ValueError("projector length differs from the compressed vector length")
+        w = np.ascontiguousarray([float(wt) for _, wt in projectors], dtype=np.float64)
+        ptrs = (_dp * max(len(vecs), 1))(*[_ptr(v.view(np.float64), C.c_double) for v in vecs])
+        _check(_LIB.cam_heff_set_projectors(self.h, len(vecs), ptrs, _ptr(w, C.c_double) if len(w) else None))
+        self._projectors = vecs
 
     def apply_device32(self, din, dout):
         """din/dout: device pointers (ints) of n complex64 (split-complex

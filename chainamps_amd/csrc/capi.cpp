@@ -256,6 +256,19 @@ int cam_heff_allowed_map(const cam_heff* hc, int64_t* idx) {
   CAM_API_END
 }
 
+int cam_heff_set_projectors(cam_heff* h, int nproj, const double* const* P, const double* weights) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(h && nproj >= 0 && (nproj == 0 || (P && weights)), "null argument");
+  set_device(h->owner);
+  std::vector<const cplx*> p(static_cast<size_t>(nproj));
+  for (int k = 0; k < nproj; ++k) {
+    CAM_REQUIRE(P[k], "null projector");
+    p[k] = reinterpret_cast<const cplx*>(P[k]);
+  }
+  h->h.set_projectors(nproj, p.data(), weights);
+  CAM_API_END
+}
+
 int cam_heff_flops(const cam_heff* h, double* flops) {
   CAM_API_BEGIN
   *flops = h->h.flops;

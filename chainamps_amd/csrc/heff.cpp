@@ -339,6 +339,47 @@ void Heff::apply_blocked(const cplx* in, cplx* out) {
   hipStream_t s = ctx->stream;
   run_stages(0, bases(0, in, out));
   for (int pi : untouched) launch_zero(out + pblocks[pi].off, pblocks[pi].size * sizeof(cplx), s);
+  if (nproj) add_projectors(in, out);
+}
+
+void Heff::set_projectors(int np, const cplx* const* P_ref, const double* weights) {
+  CAM_REQUIRE(np >= 0 && np <= 64, "at most 64 projector tensors");
+  hipStream_t s = ctx->stream;
+  nproj = 0;
+  if (np == 0) return;
+  build_perm();
+  projQ.alloc(static_cast<size_t>(n) * np);
+  DevBuf<cplx> tmp(n);
+  std::vector<cplx> q(n), w(np);
+  for (int p = 0; p < np; ++p) {
+    for (int64_t i = 0; i < n; ++i) q[i] = cplx{P_ref[p][i].re, -P_ref[p][i].im};   // Q = conj(P)
+    tmp.upload(q.data(), q.size(), s);
+    launch_scatter_map(tmp.get(), projQ.get() + static_cast<int64_t>(p) * n, perm.get(), n, s);
+    w[p] = cplx{-weights[p], 0.0};
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  projW.alloc(np);
+  projW.upload(w.data(), w.size(), s);
+  projC.alloc(64);
+  projPart.alloc(static_cast<size_t>(multi_dot_blocks(n)) * 64);
+  projQ32.alloc(static_cast<size_t>(n) * np);
+  launch_to_fp32(projQ.get(), projQ32.get(), n * np, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  nproj = np;
+}
+
+void Heff::add_projectors(const cplx* in, cplx* out) {
+  hipStream_t s = ctx->stream;
+  launch_multi_dot(projQ.get(), n, nproj, in, n, projPart.get(), projC.get(), s);
+  launch_coef_scale(projC.get(), projW.get(), nproj, s);
+  launch_multi_axpy(projQ.get(), n, nproj, projC.get(), out, n, s);
+}
+
+void Heff::add_projectors32(const float2* in, float2* out) {
+  hipStream_t s = ctx->stream;
+  launch_multi_dot32(projQ32.get(), n, nproj, in, n, projPart.get(), projC.get(), s);
+  launch_coef_scale(projC.get(), projW.get(), nproj, s);
+  launch_multi_axpy32(projQ32.get(), n, nproj, projC.get(), out, n, s);
 }
 
 void Heff::forward_stages(const cplx* in, int nstages) {
@@ -351,6 +392,7 @@ void Heff::apply_blocked32(const float2* in, float2* out) {
   hipStream_t s = ctx->stream;
   run_stages(1, bases(1, in, out));
   for (int pi : untouched) launch_zero(out + pblocks[pi].off, pblocks[pi].size * sizeof(float2), s);
+  if (nproj) add_projectors32(in, out);
 }
 
 void Heff::apply(const cplx* in, cplx* out, int precision) {

@@ -97,6 +97,17 @@ struct Heff {
   void vector_from_tensor(const BSTensor& T, cplx* x);
   // average ms per stage over `reps` matvecs (HIP events on the stream)
   void profile(int precision, int reps, double* stage_ms);
+
+  // Excited-state projector terms of TwoVertexMPOMPSMultiply
+  // (DMRG_routines.cpp:947-973): out += weight_p conj(P_p) (P_p . in), the
+  // product without conjugation.  Stored as Q_p = conj(P_p) in the block
+  // layout (n x nproj), so that c = Q^H in and out -= Q (-weight c).
+  int nproj = 0;
+  DevBuf<cplx> projQ, projW, projC, projPart;
+  DevBuf<float2> projQ32;
+  void set_projectors(int np, const cplx* const* P_ref, const double* weights);
+  void add_projectors(const cplx* in, cplx* out);
+  void add_projectors32(const float2* in, float2* out);
 };
 
 }  // namespace cam

@@ -359,6 +359,23 @@ void launch_combine(const cplx* V, int64_t ldv, int k, const cplx* c, cplx* w, i
   HIP_CHECK(hipGetLastError());
 }
 
+namespace {
+__global__ void coef_scale_kernel(double2* __restrict__ c, const double2* __restrict__ w, int k) {
+  const int i = threadIdx.x;
+  if (i < k) {
+    const double2 a = c[i], b = w[i];
+    c[i] = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  }
+}
+}  // namespace
+
+void launch_coef_scale(cplx* c, const cplx* w, int k, hipStream_t s) {
+  CAM_REQUIRE(k >= 1 && k <= kMaxK, "coef_scale: k out of range");
+  hipLaunchKernelGGL(coef_scale_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<double2*>(c),
+                     reinterpret_cast<const double2*>(w), k);
+  HIP_CHECK(hipGetLastError());
+}
+
 void launch_multi_dot32(const float2* V, int64_t ldv, int k, const float2* w, int64_t n, cplx* partial, cplx* c,
                         hipStream_t s) {
   CAM_REQUIRE(k >= 1 && k <= kMaxK, "multi_dot: k out of range");

@@ -57,6 +57,8 @@ int multi_dot_blocks(int64_t n);
 void launch_multi_dot32(const float2* V, int64_t ldv, int k, const float2* w, int64_t n, cplx* partial, cplx* c,
                         hipStream_t s);
 void launch_multi_axpy32(const float2* V, int64_t ldv, int k, const cplx* c, float2* w, int64_t n, hipStream_t s);
+// c[i] *= w[i], i < k (k <= 64), on the device
+void launch_coef_scale(cplx* c, const cplx* w, int k, hipStream_t s);
 void launch_accumulate32(const float2* V, int64_t ldv, int k, const cplx* c, cplx* w, int64_t n, hipStream_t s);
 // x[i] = (a, b), a and b uniform in [-1, 1) from a counter-based splitmix64
 // stream (deterministic for a given seed)

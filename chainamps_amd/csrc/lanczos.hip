@@ -515,7 +515,8 @@ double dense_lowest(Heff& H, cplx* xout, double* info) {
 
 double eigs_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv, bool mixed,
                    double* info) {
-  if (H.n <= kSparseThreshold) return dense_lowest(H, x, info);
+  // the dense route only without projector terms (DMRG_routines.cpp:852)
+  if (H.n <= kSparseThreshold && H.nproj == 0) return dense_lowest(H, x, info);
   return lanczos_lowest(H, x, have_start, converge, tol, ncv, mixed, info);
 }
 

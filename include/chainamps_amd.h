@@ -106,6 +106,17 @@ int cam_heff_destroy(cam_heff* h);
 /* allowed_indices of DMRG_routines.cpp:808-824 (loop order a2>s2>a0>s1):
  * n flat positions into the d*chiL*d*chiR vector. */
 int cam_heff_allowed_map(const cam_heff* h, int64_t* idx);
+/* Excited-state projector terms of TwoVertexMPOMPSMultiply
+ * (DMRG_routines.cpp:947-973; TwoVertexComponents' ProjectorTensors,
+ * DMRG_routines.cpp:781-806): every later matvec adds, for each of the
+ * nproj tensors P[k] (host, n complex in the reference compressed order,
+ * entries of the projector MPX_matrix at allowed_indices) with weight
+ * weights[k], weights[k] * (sum_i P[k][i] in[i]) * conj(P[k]) -- the
+ * contraction with the vector without conjugation, as contract00112233 --
+ * in FP64 and FP32 matvecs alike.  With projectors the eigensolver never
+ * takes the dense route (as DMRG_routines.cpp:852).  nproj = 0 removes
+ * them; at most 64. */
+int cam_heff_set_projectors(cam_heff* h, int nproj, const double* const* P, const double* weights);
 /* Algorithmic flop count of one matvec (sum of 8*m*n*k over the
  * QN-allowed block triples of the four-stage order L, W, W, R). */
 int cam_heff_flops(const cam_heff* h, double* flops);

@@ -7,7 +7,10 @@ Follows DMRG_routines.cpp:
     order a2 > sigma2 > a0 > sigma1 (808-824);
   * ``TwoVertexMPOMPSMultiply`` (898-974): scatter into V, contract
     LeftPart.V (pairs 30,41), ShiftNumRowIndices(2), contract with
-    RightPart (pairs 20,31,42), reshape to a vector, extract allowed;
+    RightPart (pairs 20,31,42), reshape to a vector, extract allowed; the
+    excited-state projector terms (947-973), out += weight (P . V) conj(P)
+    with the product taken without conjugation (pinned: with P = conj(psi0)
+    the lowest eigenvalue is the first excited one of the dense H_eff);
   * ``TwoVertexComponents::HED`` (835-896): dense route (n <= 1000,
     SPARSE_THRESHOLD, sparse_interface.hpp:16) or ARPACK znaupd with
     nev=1, which="SR", ncv=20, tol 0.1 / maxiter 2 for non-converged
@@ -57,8 +60,11 @@ def fix_complex_precision(c):
 class TwoVertexComponents:
     """DMRG_routines.hpp:189-214 / DMRG_routines.cpp:779-833."""
 
-    def __init__(self, LB: MPX, W: MPX, RB: MPX):
+    def __init__(self, LB: MPX, W: MPX, RB: MPX, projectors=None):
+        """``projectors``: (P, weight) pairs, P the projector tensor's
+        entries in the compressed order (ProjectorTensors, 781-806)."""
         self.LB, self.W, self.RB = LB, W, RB
+        self.projectors = [(np.asarray(P, dtype=np.complex128), float(w)) for P, w in (projectors or [])]
         rules = W.rules
         self.rules = rules
         lp = contract(W, False, LB, False, [(1, 3)]).remove_dummy_indices([3, 5, 6])
@@ -100,7 +106,12 @@ class TwoVertexComponents:
         SM = contract_to_sparse(M, False, self.RightPart, False, [(2, 0), (3, 1), (4, 2)])
         vec = reshape_rows(SM, self.length)
         dense = np.asarray(vec.toarray()).reshape(-1)
-        return dense[self.allowed]
+        out = dense[self.allowed]
+        # projector terms (947-973): s = contract_to_sparse(P, Vector, all
+        # four index pairs) without conjugation, out += weight s conj(P)
+        for P, w in self.projectors:
+            out = out + w * np.sum(P * vin) * np.conj(P)
+        return out
 
     def dense_matrix(self):
         """Dense H_eff on the allowed subspace, column by column through the
@@ -120,7 +131,7 @@ class TwoVertexComponents:
         """Lowest ("SR") eigenpair; returns (eigenvalue, vector of length n
         in compressed order)."""
         n = self.n
-        if n <= SPARSE_THRESHOLD:
+        if n <= SPARSE_THRESHOLD and not self.projectors:      # dense route without projectors (852)
             H = self.dense_matrix()
             w, v = np.linalg.eigh(0.5 * (H + H.conj().T))
             vec = v[:, 0]

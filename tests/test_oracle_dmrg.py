@@ -160,3 +160,27 @@ def test_prediction_fidelity_limits():
     assert abs(fids[0]) < 1e-12 and abs(fids[1]) < 1e-12
     assert all(f > 0 for f in fids[2:])
     assert fids[-1] < 0.2 * fids[2]
+
+
+def test_projector_terms_shift_the_ground_state():
+    """Projector terms (DMRG_routines.cpp:947-973): with P = conj(psi0) and
+    weight w the operator is H_eff + w |psi0><psi0|, so its lowest
+    eigenvalue is the first excited one of H_eff (dense eigh), and the
+    matvec equals the dense matrix plus that rank-one term.  The eigensolve
+    takes the ARPACK route even below SPARSE_THRESHOLD (DMRG_routines.cpp:852)."""
+    rules, phys, W = models.heisenberg_mpo()
+    idm = dmrg.IDMRG(W, converge=True)
+    for _ in range(4):
+        idm.grow(12)
+    LB = dmrg.make_left_block(idm.LB, W, idm.U)
+    RB = dmrg.make_right_block(idm.RB, W, idm.Vd)
+    comp = dmrg.TwoVertexComponents(LB, W, RB)
+    Hd = comp.dense_matrix()
+    w, v = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
+    psi0 = v[:, 0]
+    pc = dmrg.TwoVertexComponents(LB, W, RB, projectors=[(np.conj(psi0), 10.0)])
+    x = np.random.default_rng(2).standard_normal(comp.n) + 0j
+    assert np.allclose(pc.matvec(x), Hd @ x + 10.0 * psi0 * np.vdot(psi0, x), atol=1e-12)
+    E, vec = pc.hed(converge=True)
+    assert abs(E - w[1]) <= 1e-10 * max(1.0, abs(w[1]))
+    assert abs(np.vdot(psi0, vec)) <= 1e-8
