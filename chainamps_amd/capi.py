@@ -532,6 +532,8 @@ class IDMRG:
 _SIGS_ITEBD = {
     "cam_itebd_create": [_p, _p, _p, _p, _dp, C.c_int64, _dp, C.c_int64, C.c_double, C.c_int, C.POINTER(_p)],
     "cam_itebd_evolve": [_p, C.c_int64, C.c_int64, C.c_double, _dp],
+    "cam_itebd_evolve_measured": [_p, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int, C.POINTER(_p), _dp,
+                                  C.c_int64, _i64p],
     "cam_itebd_lambda": [_p, C.c_int, _dp, C.c_int64, _i64p],
     "cam_itebd_tensor": [_p, C.c_int, C.POINTER(_p)],
     "cam_itebd_destroy": [_p],
@@ -570,6 +572,25 @@ class ITEBD:
         d["steps"] = int(d["steps"])
         d["chi"] = int(d["chi"])
         return d
+
+    def evolve_measured(self, steps, D=0, minS=0.0, interval=1, measured=()):
+        """iTEBD::evolve with measurement steps: one dict per measurement
+        (step, time, truncation, entropy, abs_overlap, overlap, values)."""
+        ops = list(measured)
+        hs = (_p * max(len(ops), 1))(*[o.h for o in ops])
+        n = C.c_int64()
+        per = 7 + 2 * len(ops)
+        cap = int(steps) // max(int(interval), 1) + 1
+        rec = np.zeros(cap * per)
+        _check(_LIB.cam_itebd_evolve_measured(self.h, int(steps), int(D), float(minS), int(interval), len(ops), hs,
+                                              _ptr(rec, C.c_double), cap, C.byref(n)))
+        out = []
+        for k in range(min(n.value, cap)):
+            r = rec[k * per:(k + 1) * per]
+            out.append(dict(step=int(r[0]), time=r[1], truncation=r[2], entropy=r[3], abs_overlap=r[4],
+                            overlap=complex(r[5], r[6]),
+                            values=[complex(r[7 + 2 * j], r[8 + 2 * j]) for j in range(len(ops))]))
+        return out
 
     def lam(self, k):
         n = C.c_int64()

@@ -543,6 +543,38 @@ int cam_itebd_evolve(cam_itebd* it, int64_t nsteps, int64_t D, double minS, doub
   CAM_API_END
 }
 
+int cam_itebd_evolve_measured(cam_itebd* it, int64_t nsteps, int64_t D, double minS, int64_t interval, int nops,
+                              const cam_tensor* const* ops, double* records, int64_t cap, int64_t* nrec) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(it && nrec && nops >= 0 && (nops == 0 || ops), "null argument");
+  set_device(it->owner);
+  std::vector<TensorPtr> m;
+  for (int k = 0; k < nops; ++k) {
+    CAM_REQUIRE(ops[k], "null measured operator");
+    m.push_back(ops[k]->t);
+  }
+  const std::vector<cam::ITEBDMeasure> r = it->d->evolve_measured(nsteps, D, minS, interval, m);
+  *nrec = static_cast<int64_t>(r.size());
+  const int64_t per = 7 + 2 * nops;
+  if (records)
+    for (int64_t k = 0; k < std::min<int64_t>(cap, *nrec); ++k) {
+      double* o = records + k * per;
+      const cam::ITEBDMeasure& e = r[k];
+      o[0] = static_cast<double>(e.step);
+      o[1] = e.time;
+      o[2] = e.truncation;
+      o[3] = e.entropy;
+      o[4] = e.abs_overlap;
+      o[5] = e.overlap.re;
+      o[6] = e.overlap.im;
+      for (int j = 0; j < nops; ++j) {
+        o[7 + 2 * j] = e.values[j].re;
+        o[8 + 2 * j] = e.values[j].im;
+      }
+    }
+  CAM_API_END
+}
+
 int cam_itebd_lambda(const cam_itebd* it, int k, double* values, int64_t cap, int64_t* n) {
   CAM_API_BEGIN
   CAM_REQUIRE(k == 0 || k == 1, "iTEBD: bond index must be 0 or 1");

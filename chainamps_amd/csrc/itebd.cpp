@@ -30,6 +30,8 @@ ITEBD::ITEBD(Ctx& c, const BSTensor& W, TensorPtr A0, TensorPtr A1, std::vector<
   std::vector<int> seq;
   trotter_operators(c, W, dt, order, bond_ops, seq);
   for (int k : seq) ops.push_back(bond_ops.at(k));   // OrderedOperatorPtrs
+  initial[0] = A0;
+  initial[1] = A1;
   A[0] = std::move(A0);
   A[1] = std::move(A1);
   lam[0] = std::move(l0);
@@ -57,6 +59,84 @@ void ITEBD::canonise() {
   A[0] = n0;
   A[1] = n1;
   lam[0] = std::move(nl);
+}
+
+std::vector<ITEBDMeasure> ITEBD::evolve_measured(int64_t nsteps, int64_t D, double minS, int64_t interval,
+                                                 const std::vector<TensorPtr>& measured) {
+  CAM_REQUIRE(nsteps >= 0 && interval >= 1, "iTEBD: negative step count or measurement interval below 1");
+  std::vector<ITEBDMeasure> out;
+  auto swap_cell = [&]() {
+    std::swap(A[0], A[1]);
+    std::swap(lam[0], lam[1]);
+  };
+  auto measure = [&](int64_t step) {
+    ITEBDMeasure m;
+    m.step = step;
+    m.time = dt * static_cast<double>(step);
+    m.truncation = truncation;
+    double s = 0.0;
+    for (double x : lam[0])
+      if (x > 0.0) s -= x * x * std::log(x * x);
+    m.entropy = s;
+    m.overlap = overlap(*ctx, {initial[0], initial[1]}, {A[0], A[1]});
+    m.abs_overlap = std::hypot(m.overlap.re, m.overlap.im);
+    for (const TensorPtr& op : measured) m.values.push_back(one_vertex_measurement(*ctx, *op, A[0], A[1], lam[0]));
+    out.push_back(std::move(m));
+  };
+  auto due = [&](int64_t step) { return step % interval == 0; };
+  const int64_t s0 = steps;
+  if (order == 1) {
+    for (int64_t n = 1; n <= nsteps; ++n) {
+      gate(0, D, minS);
+      gate(1, D, minS);
+      if (due(s0 + n)) {
+        canonise();
+        measure(s0 + n);
+      }
+    }
+  } else if (order == 2) {
+    if (nsteps > 0) {
+      gate(0, D, minS);
+      for (int64_t n = 1; n < nsteps; ++n) {
+        gate(1, D, minS);
+        if (due(s0 + n)) {
+          gate(0, D, minS);
+          swap_cell();
+          canonise();
+          measure(s0 + n);
+          gate(0, D, minS);
+        } else {
+          gate(2, D, minS);
+        }
+      }
+      gate(1, D, minS);
+      gate(0, D, minS);
+      swap_cell();
+      if (due(s0 + nsteps)) {
+        canonise();
+        measure(s0 + nsteps);
+      }
+    }
+  } else {
+    gate(0, D, minS);
+    for (int64_t n = 1; n <= nsteps; ++n) {
+      for (int i = 1; i < 10; ++i) {
+        gate(i, D, minS);
+        if (i % 2) canonise();
+      }
+      if (due(s0 + n)) {
+        gate(10, D, minS);
+        swap_cell();
+        canonise();
+        measure(s0 + n);
+        gate(0, D, minS);
+      } else {
+        gate(2, D, minS);
+      }
+    }
+  }
+  steps += nsteps;
+  return out;
 }
 
 ITEBDRecord ITEBD::evolve(int64_t nsteps, int64_t D, double minS) {

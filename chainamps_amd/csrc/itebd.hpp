@@ -19,6 +19,14 @@ struct ITEBDRecord {
 // TrotterDecomposition (TEBD_routines.cpp:14-85) for orders 1, 2 and 4 and
 // the evolve loop (TEBD_routines.cpp:120-224) without measurement steps; the
 // unit cell {A0, A1, lambda0, lambda1} stays on the device.
+// do_measurements (TEBD_routines.cpp:99-118) at a measurement step
+struct ITEBDMeasure {
+  int64_t step;
+  double time, truncation, entropy, abs_overlap;
+  cplx overlap;                  // Overlap(initial cell, canonised cell)
+  std::vector<cplx> values;      // OneVertexMeasurement of each measured operator
+};
+
 struct ITEBD {
   Ctx* ctx;
   int order;
@@ -26,6 +34,7 @@ struct ITEBD {
   std::vector<TensorPtr> ops;   // OrderedOperatorPtrs
   TensorPtr A[2];
   std::vector<double> lam[2];
+  TensorPtr initial[2];          // m_initial_unit, for the overlap
   int64_t steps = 0;
   double truncation = 0.0;
 
@@ -36,6 +45,11 @@ struct ITEBD {
   // OrthogonaliseInversionSymmetric of the cell (order 4, between pieces)
   void canonise();
   ITEBDRecord evolve(int64_t nsteps, int64_t D, double minS);
+  // evolve with measurement steps every `interval` steps (of the global
+  // step count): the reference's measuring branches, canonisation by
+  // OrthogonaliseInversionSymmetric and do_measurements
+  std::vector<ITEBDMeasure> evolve_measured(int64_t nsteps, int64_t D, double minS, int64_t interval,
+                                            const std::vector<TensorPtr>& measured);
 };
 
 }  // namespace cam

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <limits>
+#include <map>
 
 #include "ortho.hpp"
 #include "schur.hpp"
@@ -733,6 +734,165 @@ cplx scalar_of(Ctx& ctx, const BSTensor& t) {
 
 }  // namespace
 
+namespace {
+
+// Charge-paired blocks of (bra index) x (ket index): one block per charge
+// present in both, rows bra, the layout of a mixed transfer-map vector.
+struct PairBond {
+  std::vector<int> bs, ks;           // bra / ket sector of each pair
+  std::vector<int64_t> nb, nk, off;
+  int64_t total = 0;
+  std::map<Charge, int> by_charge;
+};
+
+PairBond pair_bond(const QIndex& bra, const QIndex& ket) {
+  PairBond p;
+  for (int sb = 0; sb < bra.nsec(); ++sb) {
+    const int sk = ket.find_sector(bra.sec_q[sb]);
+    if (sk < 0) continue;
+    p.by_charge[bra.sec_q[sb]] = static_cast<int>(p.bs.size());
+    p.bs.push_back(sb);
+    p.ks.push_back(sk);
+    p.nb.push_back(bra.sec_size[sb]);
+    p.nk.push_back(ket.sec_size[sk]);
+    p.off.push_back(p.total);
+    p.total += bra.sec_size[sb] * ket.sec_size[sk];
+  }
+  return p;
+}
+
+// LeftComponentsMultiply with Bra != Ket (measurement.cpp:582-602): per
+// vertex W = v . A (ket) and out = B^H . W (bra), two grouped MFMA launches.
+struct MixedTransfer {
+  Ctx* ctx;
+  std::vector<TensorPtr> Pk, Pb;
+  std::vector<PairBond> bonds;
+  GemmPlan plan[2][8];
+  std::vector<std::pair<int64_t, int64_t>> empty[8];
+  DevBuf<cplx> wbuf, mid[2];
+
+  MixedTransfer(Ctx& c, const std::vector<TensorPtr>& bra, const std::vector<TensorPtr>& ket) : ctx(&c) {
+    const int L = static_cast<int>(ket.size());
+    CAM_REQUIRE(L >= 1 && L <= 8 && bra.size() == ket.size(), "Bra and Ket fragments have different sizes!");
+    for (int i = 0; i < L; ++i)
+      for (const TensorPtr& A : {bra[i], ket[i]})
+        CAM_REQUIRE(A && A->nidx() == 3 && A->idx[0].physical && A->idx[1].inward && !A->idx[2].inward,
+                    "transfer matrix: cell matrices must have indices (physical, inward, outward)");
+    CAM_REQUIRE(bra[L - 1]->idx[2].same_charges(bra[0]->idx[1]) && ket[L - 1]->idx[2].same_charges(ket[0]->idx[1]),
+                "transfer matrix: the two boundary indices of a cell carry different charges");
+    for (int i = 0; i < L; ++i) bonds.push_back(pair_bond(bra[i]->idx[1], ket[i]->idx[1]));
+    bonds.push_back(pair_bond(bra[L - 1]->idx[2], ket[L - 1]->idx[2]));
+    int64_t wtot = 1, midmax = 1;
+    std::vector<std::vector<int64_t>> woff(L);
+    for (int i = 0; i < L; ++i) {
+      Pk.push_back(reorder(c, *ket[i], false, {1, 0, 2}, 2));
+      Pb.push_back(reorder(c, *bra[i], false, {1, 0, 2}, 2));
+      int64_t w = 0;
+      const PairBond& bin = bonds[i];
+      for (const Block& B : Pk[i]->blocks) {
+        woff[i].push_back(w);
+        const Charge& qa = Pk[i]->idx[0].sec_q[B.sec[0]];
+        auto it = bin.by_charge.find(qa);
+        if (it == bin.by_charge.end()) continue;
+        w += bin.nb[it->second] * B.dims[1] * B.dims[2];
+      }
+      wtot = std::max(wtot, w);
+      if (i > 0) midmax = std::max(midmax, bonds[i].total);
+    }
+    wbuf.alloc(static_cast<size_t>(wtot));
+    if (L > 1) {
+      mid[0].alloc(static_cast<size_t>(midmax));
+      mid[1].alloc(static_cast<size_t>(midmax));
+    }
+    auto off = [](int64_t o) { return reinterpret_cast<const void*>(o); };
+    for (int i = 0; i < L; ++i) {
+      const PairBond& bin = bonds[i];
+      const PairBond& bout = bonds[i + 1];
+      const int inspace = i == 0 ? 1 : 0, outspace = i == L - 1 ? 2 : 0;
+      const cplx* inbase = i == 0 ? nullptr : mid[(i - 1) & 1].get();
+      cplx* outbase = i == L - 1 ? nullptr : mid[i & 1].get();
+      auto inptr = [&](int64_t o) { return inspace ? off(o) : static_cast<const void*>(inbase + o); };
+      auto outptr = [&](int64_t o) { return outspace ? reinterpret_cast<void*>(o) : static_cast<void*>(outbase + o); };
+      const BSTensor& K = *Pk[i];
+      const BSTensor& Bt = *Pb[i];
+      std::vector<std::vector<std::pair<int, int>>> by(bout.bs.size());   // (ket block, bra block) per output pair
+      for (size_t kb = 0; kb < K.blocks.size(); ++kb) {
+        const Block& B = K.blocks[kb];
+        const Charge& qa = K.idx[0].sec_q[B.sec[0]];
+        const Charge& qb = K.idx[2].sec_q[B.sec[2]];
+        auto ia = bin.by_charge.find(qa);
+        if (ia == bin.by_charge.end()) continue;
+        const int pa = ia->second;
+        const int64_t nbr = bin.nb[pa], nka = B.dims[0], ds = B.dims[1], nkb = B.dims[2];
+        // W[r, (s, b)] = v[r, a] A[a, (s, b)]
+        const int pi = plan[0][i].add_problem(wbuf.get() + woff[i][kb], nbr, static_cast<int>(nbr),
+                                              static_cast<int>(ds * nkb), 1, 0, 0);
+        plan[0][i].add_segment(pi, inptr(bin.off[pa]), nbr, 0, K.data.get() + B.offset, nka, 0,
+                               static_cast<int>(nka), seg_spaces(inspace, 0));
+        auto ib = bout.by_charge.find(qb);
+        if (ib == bout.by_charge.end()) continue;
+        const int bb = Bt.find_block({bin.bs[pa], B.sec[1], bout.bs[ib->second]});
+        if (bb < 0) continue;
+        by[ib->second].push_back({static_cast<int>(kb), bb});
+      }
+      for (size_t po = 0; po < bout.bs.size(); ++po) {
+        const int64_t mo = bout.nb[po], no = bout.nk[po];
+        if (!mo || !no) continue;
+        if (by[po].empty()) {
+          empty[i].push_back({bout.off[po], mo * no});
+          continue;
+        }
+        const int pi = plan[1][i].add_problem(outptr(bout.off[po]), mo, static_cast<int>(mo), static_cast<int>(no), 1,
+                                              0, 0, outspace);
+        for (const auto& kb_bb : by[po]) {
+          const Block& B = K.blocks[kb_bb.first];
+          const Block& Bb = Bt.blocks[kb_bb.second];
+          const int64_t kk = Bb.dims[0] * Bb.dims[1];   // (r, s) of the bra block
+          // out[r', b] += conj(Pb[(r, s), r']) W[(r, s), b]
+          plan[1][i].add_segment(pi, Bt.data.get() + Bb.offset, kk, 0, wbuf.get() + woff[i][kb_bb.first], kk, 0,
+                                 static_cast<int>(kk), kSegAKContig | kSegConjA);
+          (void)B;
+        }
+      }
+      for (int g = 0; g < 2; ++g)
+        if (!plan[g][i].probs.empty()) {
+          plan[g][i].choose_tiles(1);
+          plan[g][i].build_tiles();
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+
+  void apply(const cplx* in, cplx* out) {
+    const int L = static_cast<int>(Pk.size());
+    hipStream_t s = ctx->stream;
+    GemmBases b{};
+    b.p[1] = in;
+    b.p[2] = out;
+    for (int i = 0; i < L; ++i) {
+      cplx* dst = i == L - 1 ? out : mid[i & 1].get();
+      for (const auto& r : empty[i]) launch_zero(dst + r.first, r.second * static_cast<int64_t>(sizeof(cplx)), s);
+      for (int g = 0; g < 2; ++g)
+        if (!plan[g][i].probs.empty()) launch_grouped_gemm(plan[g][i], 0, s, &b);
+    }
+  }
+};
+
+}  // namespace
+
+cplx overlap(Ctx& ctx, const std::vector<TensorPtr>& bra, const std::vector<TensorPtr>& ket) {
+  MixedTransfer T(ctx, bra, ket);
+  const int64_t n = T.bonds.front().total;
+  CAM_REQUIRE(n > 0, "Overlap: no common charge sector");
+  CAM_REQUIRE(T.bonds.back().total == n, "Index dimensions mismatch between ends of translation unit!");
+  DevBuf<cplx> x(n);
+  launch_random_fill(x.get(), n, 0x5eedull, ctx.stream);
+  const LeadingEig e = leading_eigenpair(
+      ctx, [&](const cplx* in, cplx* out) { T.apply(in, out); }, n, x.get(), kNcv, kMaxRestarts, 1e-14);
+  CAM_REQUIRE(e.converged || e.residual <= 1e-10, "Overlap: the transfer-matrix eigensolve did not converge");
+  return e.value;
+}
+
 TensorPtr unit_cell_second(Ctx& ctx, const BSTensor& Vd, const std::vector<double>& values,
                            const std::vector<double>& previous) {
   CAM_REQUIRE(Vd.nidx() == 3, "UnitCell: the row matrix must have three indices");
@@ -816,6 +976,19 @@ void orthogonalise_inversion_symmetric(Ctx& ctx, const TensorPtr& A0, const Tens
   info.seconds = seconds_since(t0);
 }
 
+cplx one_vertex_measurement(Ctx& ctx, const BSTensor& Op, const TensorPtr& A0, const TensorPtr& A1,
+                            const std::vector<double>& lambda0) {
+  CAM_REQUIRE(Op.nidx() == 4, "OneVertexMeasurement: the operator must be an MPO (four indices)");
+  std::vector<double> l2(lambda0.size());
+  for (size_t k = 0; k < l2.size(); ++k) l2[k] = lambda0[k] * lambda0[k];
+  const TensorPtr L2 = diagonal_tensor(ctx, A0->idx[1], l2, A0->rules);
+  TensorPtr t = contract(ctx, *A0, true, *A0, false, {{0, 0}, {1, 1}});
+  t = contract(ctx, *t, false, *A1, false, {{1, 1}});
+  t = contract(ctx, *t, false, Op, false, {{1, 2}});
+  t = contract(ctx, *A1, true, *t, false, {{0, 2}, {1, 0}});
+  return scalar_of(ctx, *contract(ctx, *t, false, *L2, false, {{0, 1}, {1, 0}}));
+}
+
 cplx i_two_vertex_energy(Ctx& ctx, const BSTensor& W, const TensorPtr& A0, const TensorPtr& A1,
                          const std::vector<double>& lambda0, cplx terms[3]) {
   CAM_REQUIRE(W.nidx() == 4, "iTwoVertexEnergy: the Hamiltonian must be an MPO (four indices)");
@@ -841,11 +1014,7 @@ cplx i_two_vertex_energy(Ctx& ctx, const BSTensor& W, const TensorPtr& A0, const
     t = c(A1, true, t, false, {{0, 3}, {1, 0}});
     local += close(t, {{0, 1}, {3, 0}});
     // OneVertexMeasurement: H1 on the second vertex
-    t = c(A0, true, A0, false, {{0, 0}, {1, 1}});
-    t = c(t, false, A1, false, {{1, 1}});
-    t = c(t, false, H1, false, {{1, 2}});
-    t = c(A1, true, t, false, {{0, 2}, {1, 0}});
-    local += close(t, {{0, 1}, {1, 0}});
+    local += to_z(one_vertex_measurement(ctx, *H1, A0, A1, lambda0));
   }
   if (w >= 3) {
     const TensorPtr ColX = extract_mpo_block(ctx, W, 1, w - 2, 0, 0);

@@ -97,6 +97,17 @@ void orthogonalise_inversion_symmetric(Ctx& ctx, const TensorPtr& A0, const Tens
                                        const std::vector<double>& lambda0, TensorPtr& A0n, TensorPtr& A1n,
                                        std::vector<double>& lambda0n, OrthoInfo& info);
 
+// Overlap(bra, ket) (measurement.cpp:282-288): the largest-magnitude
+// eigenvalue of TransferMatrixComponents(bra, ket, 0, State(rules)), the
+// left map v -> sum_s B_s^H v A_s on the charge-paired blocks of (bra
+// inward) x (ket inward).
+cplx overlap(Ctx& ctx, const std::vector<TensorPtr>& bra, const std::vector<TensorPtr>& ket);
+
+// OneVertexMeasurement (measurement.cpp:317-324): the MPO operator Op on the
+// second vertex of the cell, closed with lambda0^2.
+cplx one_vertex_measurement(Ctx& ctx, const BSTensor& Op, const TensorPtr& A0, const TensorPtr& A1,
+                            const std::vector<double>& lambda0);
+
 // iTwoVertexEnergy (measurement.cpp:371-397) of the orthogonalised cell
 // [A0, A1] with boundary lambda lambda0, from the Hamiltonian MPO W.
 // terms: on-vertex, bond inside the cell, bond to the next cell.

@@ -255,6 +255,19 @@ int cam_itebd_create(cam_ctx* ctx, const cam_tensor* W, const cam_tensor* A0, co
                      int order, cam_itebd** out);
 int cam_itebd_evolve(cam_itebd* it, int64_t nsteps, int64_t D, double minS, double* record);
 /* bond weights k (0 or 1), capacity cap; *n receives the count */
+/* iTEBD::evolve with measurement steps (TEBD_routines.cpp:120-231): every
+ * `interval` steps of the object's step count the reference's measuring
+ * branch runs (order 2: odd(dt/2), cell swap, canonisation, odd(dt/2);
+ * order 4: piece 10, swap, canonisation, piece 0), the cell is canonised
+ * with OrthogonaliseInversionSymmetric and do_measurements (99-118) records
+ * 7 + 2 nops doubles: {step, time, truncation of the last gate, entropy of
+ * lambda0, |overlap|, overlap (re, im)} with the overlap Overlap(initial
+ * cell, cell) (measurement.cpp:282-288, a bra != ket transfer-matrix
+ * eigenvalue), then OneVertexMeasurement (317-324) of each of the nops MPO
+ * operators ops[k] as (re, im).  *nrec receives the number of records; up
+ * to cap are written. */
+int cam_itebd_evolve_measured(cam_itebd* it, int64_t nsteps, int64_t D, double minS, int64_t interval, int nops,
+                              const cam_tensor* const* ops, double* records, int64_t cap, int64_t* nrec);
 int cam_itebd_lambda(const cam_itebd* it, int k, double* values, int64_t cap, int64_t* n);
 /* a copy of cell tensor k (0 or 1) as a new tensor handle */
 int cam_itebd_tensor(const cam_itebd* it, int k, cam_tensor** out);

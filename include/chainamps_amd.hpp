@@ -410,6 +410,25 @@ class iTEBD {
     check(cam_itebd_evolve(it_.get(), num_steps, bond_dimension, minS, r.data()));
     return r;
   }
+  /* iTEBD::evolve(num_steps, measuredMPOs, bond_dimension, minS,
+   * measurement_interval) (TEBD_routines.cpp:120-231): one row per
+   * measurement, {step, time, truncation, entropy, |overlap|, overlap re,
+   * overlap im, then (re, im) of OneVertexMeasurement of each operator}. */
+  std::vector<std::vector<double>> evolve(MPXInt num_steps, const std::vector<MPX_matrix>& measuredMPOs,
+                                          MPXInt bond_dimension, double minS, MPXInt measurement_interval) {
+    std::vector<const cam_tensor*> ops;
+    for (const MPX_matrix& m : measuredMPOs) ops.push_back(m.handle());
+    const int64_t per = 7 + 2 * static_cast<int64_t>(ops.size());
+    const int64_t cap = num_steps / std::max<MPXInt>(measurement_interval, 1) + 1;
+    std::vector<double> flat(static_cast<size_t>(cap * per));
+    int64_t n = 0;
+    check(cam_itebd_evolve_measured(it_.get(), num_steps, bond_dimension, minS, measurement_interval,
+                                    static_cast<int>(ops.size()), ops.data(), flat.data(), cap, &n));
+    std::vector<std::vector<double>> rows;
+    for (int64_t k = 0; k < std::min(n, cap); ++k)
+      rows.emplace_back(flat.begin() + k * per, flat.begin() + (k + 1) * per);
+    return rows;
+  }
   std::vector<double> Lambda(int k) const {
     int64_t n = 0;
     check(cam_itebd_lambda(it_.get(), k, nullptr, 0, &n));

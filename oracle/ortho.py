@@ -124,6 +124,43 @@ def right_apply(dense_cell, v):
     return v
 
 
+def mixed_left_apply(bra_cell, ket_cell, v):
+    """LeftComponentsMultiply with Bra != Ket (measurement.cpp:582-602):
+    v -> sum_s B_s^H v A_s over the cells, v rows on the bra index."""
+    for B, A in zip(bra_cell, ket_cell):
+        v = np.einsum("sab,ac,scd->bd", B.conj(), v, A)
+    return v
+
+
+def overlap(bra_mats, ket_mats, ncv=NCV):
+    """Overlap(bra, ket) (measurement.cpp:282-288): the largest-magnitude
+    eigenvalue of TransferMatrixComponents(bra, ket, 0, State(rules)), i.e.
+    of the left map on the entries (r, c) of (bra inward) x (ket inward)
+    with equal charges (no Hermitian completion)."""
+    B = [_dense(m) for m in bra_mats]
+    A = [_dense(m) for m in ket_mats]
+    qb, qk = bra_mats[0].indices[1].charges, ket_mats[0].indices[1].charges
+    nb, nk = qb.shape[0], qk.shape[0]
+    if nb * nk != bra_mats[-1].indices[2].size * ket_mats[-1].indices[2].size:
+        raise ValueError("Index dimensions mismatch between ends of translation unit!")
+    same = np.all(qb[:, None, :] == qk[None, :, :], axis=2)         # same[r, c]
+    cc, rr = np.nonzero(same.T)                                      # c outer, r inner
+    m = len(rr)
+
+    def mv(x):
+        V = np.zeros((nb, nk), complex)
+        V[rr, cc] = x
+        return mixed_left_apply(B, A, V)[rr, cc]
+
+    if m <= ncv + 1:
+        T = np.stack([mv(e) for e in np.eye(m, dtype=complex)], axis=1)
+        w = np.linalg.eigvals(T)
+        return complex(w[int(np.argmax(np.abs(w)))])
+    op = spla.LinearOperator((m, m), matvec=mv, dtype=np.complex128)
+    w = spla.eigs(op, k=1, which="LM", ncv=min(ncv, m - 1), tol=0.0, return_eigenvectors=False)
+    return complex(w[0])
+
+
 def transfer_leading(mats, side, ncv=NCV, tol=0.0):
     """LeftED / RightED with NUMEVALS = 1, LARGESTMAGNITUDE
     (measurement.cpp:451-580).  Returns (eigenvalue, completed Hermitian

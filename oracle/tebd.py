@@ -107,6 +107,91 @@ def trotter_operators(W: MPX, tau: float, order: int):
     raise ValueError("Only 1st, 2nd and 4th order decompositions are supported")
 
 
+def evolve_measured(ops, order, cell, steps, D, minS, interval, measured=(), initial=None):
+    """iTEBD::evolve (TEBD_routines.cpp:120-231) with measurement steps every
+    `interval` steps (m_current_time_step counted from 1 in this call; the
+    time step tau enters through ``ops``).  At a measurement step the cell is
+    canonised with OrthogonaliseInversionSymmetric (measurement.cpp:131-178)
+    and do_measurements (99-118) records {step, truncation of the last gate,
+    entropy of the new lambda0, |overlap|, overlap with the initial cell,
+    OneVertexMeasurement of each measured operator}.  Order 2 follows the
+    reference's measuring branch (odd(tau/2), swap, canonise, odd(tau/2)) and
+    uses the odd full step [2] between steps elsewhere (see evolve)."""
+    from .ortho import orthogonalise_inversion_symmetric, overlap, one_vertex_measurement
+    A0, A1, l0, l1 = cell
+    init = [A0, A1] if initial is None else initial
+    tr = 0.0
+    records = []
+
+    def gate(G):
+        nonlocal A0, A1, l0, l1, tr
+        nA0, nA1, nl0, tr = apply_and_decompose(G, A0, A1, l0, D, minS)
+        A0, A1, l0, l1 = nA0, nA1, nl0, l0
+
+    def canonise():
+        nonlocal A0, A1, l0, l1
+        (A0, A1), (l0, l1), _ = orthogonalise_inversion_symmetric([A0, A1], [l0, l1])
+
+    def measure(step):
+        ov = overlap(init, [A0, A1])
+        v = np.asarray(l0, float)
+        nz = v[v > 0]
+        rec = dict(step=step, truncation=tr, entropy=float(-np.sum(nz * nz * np.log(nz * nz))),
+                   abs_overlap=abs(ov), overlap=ov,
+                   values=[one_vertex_measurement(op, [A0, A1], [l0, l1]) for op in measured])
+        records.append(rec)
+
+    def swap():
+        nonlocal A0, A1, l0, l1
+        A0, A1, l0, l1 = A1, A0, l1, l0
+
+    due = lambda n: interval > 0 and n % interval == 0
+    if order == 1:
+        for n in range(1, steps + 1):
+            gate(ops[0])
+            gate(ops[1])
+            if due(n):
+                canonise()
+                measure(n)
+    elif order == 2:
+        if steps > 0:
+            gate(ops[0])
+            for n in range(1, steps):
+                gate(ops[1])
+                if due(n):
+                    gate(ops[0])
+                    swap()
+                    canonise()
+                    measure(n)
+                    gate(ops[0])
+                else:
+                    gate(ops[2])
+            gate(ops[1])
+            gate(ops[0])
+            swap()
+            if due(steps):
+                canonise()
+                measure(steps)
+    elif order == 4:
+        gate(ops[0])
+        for n in range(1, steps + 1):
+            for i in range(1, 10):
+                gate(ops[i])
+                if i % 2:
+                    canonise()
+            if due(n):
+                gate(ops[10])
+                swap()
+                canonise()
+                measure(n)
+                gate(ops[0])
+            else:
+                gate(ops[2])
+    else:
+        raise ValueError("order 1, 2 or 4")
+    return [A0, A1, l0, l1], tr, records
+
+
 def evolve(ops, order, cell, steps, D=0, minS=0.0):
     """iTEBD::evolve (TEBD_routines.cpp:120-224) for orders 1, 2 and 4 with no
     measurement step inside the run.  ``cell`` = [A0, A1, lambda0, lambda1];

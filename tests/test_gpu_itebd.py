@@ -87,3 +87,31 @@ def test_itebd_order_four_matches_oracle(ctx):
     o, dinfo = c.orthogonalise(inversion_symmetric=True)
     assert np.allclose(np.sort(o.lam(0)), np.sort(nl[0]), rtol=1e-8, atol=1e-11)
     assert abs(dinfo["entropy"] - info["entropy"]) < 1e-9
+
+
+@pytest.mark.parametrize("order", [1, 2, 4])
+def test_itebd_measurements_match_oracle(ctx, order):
+    """Measurement steps (TEBD_routines.cpp:99-231): canonisation and
+    do_measurements at steps 2 and 4 -- the overlap with the initial cell
+    (a bra != ket transfer-matrix eigenvalue, 1e-8), entropy and truncation
+    (1e-8) and the occupation of level 0 (OneVertexMeasurement, 1e-8) -- and
+    the closed form exp(-2 i E_0 t) without the coupling (1e-10)."""
+    print(f"case itebd measurements order {order}", flush=True)
+    import chainamps_amd as cam
+    from test_oracle_tebd import _level_projector
+    for Jp in (0.0, 0.4):
+        rules, phys, W, E, sig = models.coupled_chain_model(d=4, seed=2, Jperp=Jp)
+        A0, A1, l0, l1 = tebd.product_state_cell(rules, phys, 0)
+        P0 = _level_projector(rules, phys, 0)
+        dt = 0.1
+        ops = tebd.trotter_operators(W, dt, order)
+        cell, tr, recs = tebd.evolve_measured(ops, order, [A0, A1, l0, l1], 4, 16, 1e-12, 2, [P0])
+        dev = cam.ITEBD(to_lib(ctx, W), to_lib(ctx, A0), to_lib(ctx, A1), l0, l1, dt, order)
+        got = dev.evolve_measured(4, 16, 1e-12, 2, [to_lib(ctx, P0)])
+        assert [g["step"] for g in got] == [r["step"] for r in recs]
+        for g, r in zip(got, recs):
+            assert abs(g["overlap"] - r["overlap"]) <= 1e-8, (Jp, g, r)
+            assert abs(g["entropy"] - r["entropy"]) <= 1e-8
+            assert abs(g["values"][0] - r["values"][0]) <= 1e-8
+            if Jp == 0.0:
+                assert abs(g["overlap"] - np.exp(-2j * E[0] * g["time"])) <= 1e-10

@@ -92,3 +92,43 @@ def test_inversion_symmetric_orthogonalisation_is_left_canonical():
     assert np.allclose(ortho.left_apply(dense, np.eye(m)), np.eye(m), atol=1e-8)
     assert abs(np.sum(np.asarray(nl[0]) ** 2) - 1.0) < 1e-12
     assert np.array_equal(nl[1], c[3])
+
+
+def _level_projector(rules, phys, k):
+    """|k><k| as a one-vertex MPO operator (MPO bond dimension 1)."""
+    from oracle.mpx import MPX
+    from oracle.qn import MPXIndex
+    d = phys.shape[0]
+    z = np.zeros((1, len(rules.rules)), dtype=np.int64)
+    T = np.zeros((d, 1, d, 1), complex)
+    T[k, 0, k, 0] = 1.0
+    idx = [MPXIndex(True, phys, True), MPXIndex(True, z), MPXIndex(False, phys, True), MPXIndex(False, z)]
+    return MPX.from_dense_tensor(rules, idx, 2, T)
+
+
+def test_measured_itebd_closed_form():
+    """Measurement steps (TEBD_routines.cpp:99-231): without the inter-chain
+    coupling the product state of level 0 is an eigenstate, so every
+    measurement sees the overlap per two-vertex cell exp(-2 i E_0 t), zero
+    entropy and occupation 1 of level 0 -- at orders 1, 2 and 4."""
+    rules, phys, W, E, sig = models.coupled_chain_model(d=4, seed=2, Jperp=0.0)
+    cell0 = list(tebd.product_state_cell(rules, phys, 0))
+    P0 = _level_projector(rules, phys, 0)
+    dt = 0.1
+    for order in (1, 2, 4):
+        ops = tebd.trotter_operators(W, dt, order)
+        _, _, recs = tebd.evolve_measured(ops, order, list(cell0), 4, 16, 1e-12, 2, [P0])
+        assert [r["step"] for r in recs] == [2, 4]
+        for r in recs:
+            assert abs(r["overlap"] - np.exp(-2j * E[0] * r["step"] * dt)) <= 1e-12, (order, r)
+            assert abs(r["entropy"]) <= 1e-12
+            assert abs(r["values"][0] - 1.0) <= 1e-12
+
+
+def test_overlap_of_a_canonical_cell_with_itself_is_one():
+    from oracle import ortho
+    rules, phys, W, E, sig = models.coupled_chain_model(d=4, seed=2, Jperp=0.4)
+    c, _ = tebd.evolve(tebd.trotter_operators(W, 0.1, 2), 2, list(tebd.product_state_cell(rules, phys, 0)), 3, 16,
+                       1e-12)
+    new, nl, info = ortho.orthogonalise_inversion_symmetric([c[0], c[1]], [c[2], c[3]])
+    assert abs(ortho.overlap(new, new) - 1.0) <= 1e-10
