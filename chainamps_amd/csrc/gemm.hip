@@ -447,9 +447,16 @@ int GemmPlan::add_problem(void* C, int64_t csC, int M, int N, int batch, int64_t
   p.tile = 0;
   p.cspace = cspace;
   probs.push_back(p);
+  group.push_back(-1);
   uploaded = false;
   for (int m = 0; m < kTileModes; ++m) built[m] = tiles_uploaded[m] = false;
   return static_cast<int>(probs.size()) - 1;
+}
+
+void GemmPlan::set_group(int prob, int64_t g) {
+  CAM_REQUIRE(prob >= 0 && prob < static_cast<int>(probs.size()), "set_group: no such problem");
+  group[prob] = g;
+  for (int m = 0; m < kTileModes; ++m) built[m] = tiles_uploaded[m] = false;
 }
 
 void GemmPlan::add_segment(int prob, const void* A, int64_t csA, int64_t bsA, const void* B,
@@ -520,7 +527,11 @@ void GemmPlan::build_tiles_mode(int mode) {
   // columns (problems whose first segment has the same B operand, e.g. the
   // three MPO channels applied to one psi block) next to each other, M-tiles
   // innermost: consecutive tiles run at the same time on one XCD, so B is
-  // fetched from HBM once and re-read from that XCD's L2.
+  // fetched from HBM once and re-read from that XCD's L2.  A plan with row
+  // groups orders by (group, batch, m0) first instead: the tiles that read
+  // the same A rows run together (such plans have far more tiles than the
+  // grid, so the cost order matters less than the shared fetch).
+  const bool grouped = std::any_of(group.begin(), group.end(), [](int64_t g) { return g >= 0; });
   auto bkey = [&](const GemmTile& g) -> uint64_t {
     const GemmProb& p = probs[g.prob];
     if (!p.nseg) return 0;
@@ -538,6 +549,15 @@ void GemmPlan::build_tiles_mode(int mode) {
       bk[i] = bkey(T[t][i]);
     }
     std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+      if (grouped) {
+        const GemmTile &x = T[t][a], &y = T[t][b];
+        const int64_t gx = group[x.prob], gy = group[y.prob];
+        if (gx != gy) return gx < gy;
+        if (x.batch != y.batch) return x.batch < y.batch;
+        if (x.m0 != y.m0) return x.m0 < y.m0;
+        if (x.prob != y.prob) return x.prob < y.prob;
+        return x.n0 < y.n0;
+      }
       if (cls[a] != cls[b]) return cls[a] > cls[b];
       if (bk[a] != bk[b]) return bk[a] < bk[b];
       if (T[t][a].n0 != T[t][b].n0) return T[t][a].n0 < T[t][b].n0;
@@ -574,6 +594,7 @@ void GemmPlan::upload(hipStream_t s) {
 void GemmPlan::clear() {
   probs.clear();
   segs.clear();
+  group.clear();
   for (int m = 0; m < kTileModes; ++m) {
     for (auto& t : tiles[m]) t.clear();
     built[m] = false;

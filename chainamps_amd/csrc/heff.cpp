@@ -209,11 +209,18 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     st[0].add_segment(pi, off(e.off), e.ld, 0, off(Pb.off), Pb.m0, 0, static_cast<int>(Pb.m0),
                       seg_spaces(kSpL, kSpIn));
   }
+  // W stages: the outputs of every MPO channel and new sector with one row
+  // space read the same operand rows (row group)
+  std::map<std::array<int, 3>, int64_t> rgB, rgC;
+  auto row_group = [](std::map<std::array<int, 3>, int64_t>& m, std::array<int, 3> k) {
+    return m.emplace(k, static_cast<int64_t>(m.size())).first->second;
+  };
   for (auto& kv : segB) {
     const TB& tb = t2.at(kv.first);
     const int64_t rows = tb.m0 * tb.m3;
     const int pi = st[1].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n1),
                                      static_cast<int>(tb.n2), rows * tb.n1, 0, kSpT2);
+    st[1].set_group(pi, row_group(rgB, {kv.first[1], kv.first[2], kv.first[4]}));
     for (const SegW& sg : kv.second) {
       const TB& sb = t1.at(sg.src);
       st[1].add_segment(pi, off(sb.off), rows, rows * sb.n1, off(sg.W.off), sg.W.ld, 0,
@@ -225,6 +232,7 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     const int64_t rows = tb.m0 * tb.m3 * tb.n1;
     const int pi = st[2].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n2),
                                      1, 0, 0, kSpT3);
+    st[2].set_group(pi, row_group(rgC, {kv.first[1], kv.first[2], kv.first[3]}));
     for (const SegW& sg : kv.second) {
       const TB& sb = t2.at(sg.src);
       st[2].add_segment(pi, off(sb.off), rows, 0, off(sg.W.off), sg.W.ld, 0, static_cast<int>(sb.n2),
