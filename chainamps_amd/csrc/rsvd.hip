@@ -19,6 +19,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "solvers.hpp"
 
@@ -176,6 +179,13 @@ void orthonormalise(Ctx& ctx, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& G, De
   householder_orthonormalise(static_cast<rocblas_handle>(ctx.blas_handle()), X, m, k, tau);
 }
 
+// Rounds the last accepted decomposition of each (device, m, n, k) needed:
+// the first residual check (a Rayleigh-Ritz projection, a k x k core and
+// the residuals, about 1.5 rounds' worth) waits until then, since the
+// blocks of consecutive iDMRG / iTEBD steps converge alike.
+std::mutex g_rounds_mu;
+std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> g_rounds;
+
 }  // namespace
 
 bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int64_t ncheck, double tol,
@@ -192,6 +202,15 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   launch_random_fill(W.get(), n * k, 0x5eed0000ull + static_cast<uint64_t>(m * 31 + n), s);
   gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
   int q = 0;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  const auto shape = std::make_tuple(dev, m, n, k);
+  int first = (m * n > (int64_t(1) << 24)) ? 4 : 2;
+  {
+    std::lock_guard<std::mutex> lk(g_rounds_mu);
+    auto it = g_rounds.find(shape);
+    if (it != g_rounds.end()) first = std::max(first, std::min(it->second, max_rounds));
+  }
   // CAM_SVD_TRACE=1: time of the subspace rounds, the Rayleigh-Ritz
   // projection, the k x k core and the residual check of every check
   const char* tv = std::getenv("CAM_SVD_TRACE");
@@ -206,10 +225,10 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   double t_round = 0, t_proj = 0, t_core = 0, t_res = 0;
   auto tp = std::chrono::steady_clock::now();
   for (;;) {
-    // four subspace rounds before the first residual check on blocks of more
-    // than 2^24 elements (they have needed at least six), then two between
-    // checks
-    const int nr = (q == 0 && m * n > (int64_t(1) << 24)) ? 4 : 2;
+    // before the first residual check: four subspace rounds on blocks of
+    // more than 2^24 elements (they have needed at least six), else two, or
+    // the rounds the last block of this shape needed; then two between checks
+    const int nr = q == 0 ? first : 2;
     for (int r = 0; r < nr; ++r, ++q) {
       orthonormalise(ctx, Y.get(), m, k, G, dinfo, tau);
       gemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, k, m, A, m, Y.get(), m, W.get(), n);
@@ -260,7 +279,11 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
                    static_cast<long long>(m), static_cast<long long>(n), static_cast<long long>(k), q, worst, tol,
                    t_round, t_proj, t_core, t_res);
     if (rounds) *rounds = q;
-    if (worst <= tol) return true;
+    if (worst <= tol) {
+      std::lock_guard<std::mutex> lk(g_rounds_mu);
+      g_rounds[shape] = q;
+      return true;
+    }
     if (q >= max_rounds) return false;
     // continue iterating from the current subspace
     HIP_CHECK(hipMemcpyAsync(Y.get(), U.get(), m * k * sizeof(cplx), hipMemcpyDeviceToDevice, s));
