@@ -235,7 +235,7 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   DevBuf<int> dinfo(1);
   DevBuf<double> dE, dS;
-  DevBuf<cplx> partial(64 * 1024), dcoef(64);
+  DevBuf<cplx> partial(static_cast<size_t>(multi_dot_blocks(int64_t(1) << 40))), dcoef(64);   // k = 1 below
   const char* ev = std::getenv("CAM_SVD_EXACT");
   const bool exact_only = ev && *ev && *ev != '0';
   // CAM_SVD_TRACE=1 prints the path and time of every decomposed block
