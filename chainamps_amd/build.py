@@ -57,7 +57,10 @@ def compile_one(src: Path, opt: list[str]) -> Path:
 
 def build(jobs: int = 0, debug: bool = False) -> Path:
     LIBDIR.mkdir(exist_ok=True)
-    OBJDIR.mkdir(exist_ok=True)
+    # objects of deleted or renamed sources must not survive a build
+    if OBJDIR.exists():
+        shutil.rmtree(OBJDIR)
+    OBJDIR.mkdir()
     opt = ["-O0", "-g"] if debug else ["-O3", "-DNDEBUG"]
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 1)

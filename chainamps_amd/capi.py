@@ -74,6 +74,10 @@ _SIGS = {
     "cam_heff_apply_blocked32": [_p, _p, _p],
     "cam_heff_set_projectors": [_p, C.c_int, C.POINTER(_dp), _dp],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
+    "cam_heff_profile_device": [_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp],
+    "cam_heff_set_slices": [_p, C.c_int, C.c_int],
+    "cam_heff_set_workspace_limit": [_p, C.c_double],
+    "cam_heff_get_slices": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp],
     "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
     "cam_heff_stage_bytes": [_p, _dp],
@@ -348,6 +352,29 @@ class Heff:
         _check(_LIB.cam_heff_profile(self.h, precision, reps, _ptr(ms, C.c_double), _ptr(fl, C.c_double)))
         return ms, fl
 
+    def profile_device(self, din, dout, precision=CAM_FP64, reps=5):
+        """As profile(), on the caller's device block-layout vectors."""
+        ms = np.zeros(4)
+        fl = np.zeros(4)
+        _check(_LIB.cam_heff_profile_device(self.h, precision, reps, din, dout, _ptr(ms, C.c_double),
+                                            _ptr(fl, C.c_double)))
+        return ms, fl
+
+    def set_slices(self, row_slices, col_slices):
+        """Run the matvec in row_slices x col_slices slices of the
+        intermediates (1 x 1: unsliced)."""
+        _check(_LIB.cam_heff_set_slices(self.h, int(row_slices), int(col_slices)))
+
+    def set_workspace_limit(self, nbytes):
+        """Fewest slices whose intermediates fit nbytes."""
+        _check(_LIB.cam_heff_set_workspace_limit(self.h, float(nbytes)))
+
+    def slices(self):
+        """(row slices, column slices, workspace bytes)."""
+        r, c, b = C.c_int(), C.c_int(), C.c_double()
+        _check(_LIB.cam_heff_get_slices(self.h, C.byref(r), C.byref(c), C.byref(b)))
+        return r.value, c.value, b.value
+
     def stage_bytes(self):
         """Algorithmic HBM bytes of each FP64 stage (L, W, W, R)."""
         b = np.zeros(4)
@@ -474,12 +501,23 @@ _SIGS_IDMRG = {
     "cam_idmrg_grow": [_p, C.c_int64, C.c_double, _dp],
     "cam_idmrg_values": [_p, _dp, C.c_int64, _i64p],
     "cam_idmrg_destroy": [_p],
+    "cam_idmrg_move": [_p, C.c_int, C.c_int64, C.c_double, _dp],
+    "cam_fdmrg_sweeps": [_p, C.c_int64, C.c_int64, C.c_double, _dp, C.POINTER(C.c_int32), C.c_int64, _i64p],
+    "cam_idmrg_blocks": [_p, C.POINTER(_p), C.POINTER(_p)],
 }
 _SIGS.update(_SIGS_IDMRG)
 
 IDMRG_FIELDS = ("N", "energy", "energy_per_vertex", "entropy", "truncation", "chi", "n",
                 "matvecs_fp64", "matvecs_fp32", "residual", "seconds", "heff_flops",
-                "eig_seconds", "svd_seconds", "block_seconds", "fidelity")
+                "eig_seconds", "svd_seconds", "block_seconds", "fidelity", "guess_dropped", "left", "right")
+
+
+def _idmrg_record(rec):
+    d = dict(zip(IDMRG_FIELDS, np.asarray(rec).tolist()))
+    for k in ("N", "chi", "n", "matvecs_fp64", "matvecs_fp32", "left", "right"):
+        d[k] = int(d[k])
+    d["guess_dropped"] = bool(d["guess_dropped"])
+    return d
 
 
 class IDMRG:
@@ -503,12 +541,35 @@ class IDMRG:
         return UnitCell(self.ctx, h)
 
     def grow(self, chi, truncation=0.0):
-        rec = np.zeros(16)
+        rec = np.zeros(len(IDMRG_FIELDS))
         _check(_LIB.cam_idmrg_grow(self.h, int(chi), float(truncation), _ptr(rec, C.c_double)))
-        d = dict(zip(IDMRG_FIELDS, rec.tolist()))
-        for k in ("N", "chi", "n", "matvecs_fp64", "matvecs_fp32"):
-            d[k] = int(d[k])
-        return d
+        return _idmrg_record(rec)
+
+    def move(self, right, chi, truncation=0.0):
+        """One finite step: SuperBlock::move_right_two_vertex (right=True)
+        or move_left_two_vertex (DMRG_routines.cpp:376-444)."""
+        rec = np.zeros(len(IDMRG_FIELDS))
+        _check(_LIB.cam_idmrg_move(self.h, int(bool(right)), int(chi), float(truncation), _ptr(rec, C.c_double)))
+        return _idmrg_record(rec)
+
+    def finite_sweeps(self, num_sweeps, chi, truncation=0.0):
+        """FiniteDMRG::run (fDMRG_routines.cpp:19-66): (every step's record,
+        the records the reference writes to Energies.dat)."""
+        n = C.c_int64()
+        _check(_LIB.cam_fdmrg_sweeps(self.h, int(num_sweeps), int(chi), float(truncation), None, None, 0,
+                                     C.byref(n)))
+        recs = np.zeros((max(n.value, 1), len(IDMRG_FIELDS)))
+        mid = np.zeros(max(n.value, 1), np.int32)
+        _check(_LIB.cam_fdmrg_sweeps(self.h, int(num_sweeps), int(chi), float(truncation), _ptr(recs, C.c_double),
+                                     _ptr(mid, C.c_int32), n.value, C.byref(n)))
+        steps = [_idmrg_record(r) for r in recs[:n.value]]
+        return steps, [r for r, m in zip(steps, mid[:n.value]) if m]
+
+    def blocks(self):
+        """The left and right blocks of the current centre."""
+        l, r = _p(), _p()
+        _check(_LIB.cam_idmrg_blocks(self.h, C.byref(l), C.byref(r)))
+        return Tensor(self.ctx, l), Tensor(self.ctx, r)
 
     def values(self):
         n = C.c_int64()

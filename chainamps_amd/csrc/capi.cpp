@@ -301,6 +301,46 @@ int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, dou
   CAM_API_END
 }
 
+int cam_heff_profile_device(cam_heff* h, int precision, int reps, const void* in, void* out, double* stage_ms,
+                            double* stage_flops) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  CAM_REQUIRE(in && out, "cam_heff_profile_device: null vector");
+  h->h.profile(precision, reps, stage_ms, in, out);
+  if (stage_flops)
+    for (int i = 0; i < 4; ++i) stage_flops[i] = h->h.stage_flops[i];
+  CAM_API_END
+}
+
+int cam_heff_set_slices(cam_heff* h, int row_slices, int col_slices) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(row_slices >= 1 && row_slices <= 4096 && col_slices >= 1 && col_slices <= 4096,
+              "slice counts must be 1..4096");
+  h->h.plan_slices(row_slices, col_slices);
+  HIP_CHECK(hipStreamSynchronize(h->owner->ctx.stream));
+  CAM_API_END
+}
+
+int cam_heff_set_workspace_limit(cam_heff* h, double bytes) {
+  CAM_API_BEGIN
+  set_device(h->owner);
+  CAM_REQUIRE(bytes > 0.0, "workspace limit must be positive");
+  h->h.choose_slices(bytes);
+  h->h.plan_slices(h->h.rslices, h->h.cslices);
+  HIP_CHECK(hipStreamSynchronize(h->owner->ctx.stream));
+  CAM_API_END
+}
+
+int cam_heff_get_slices(const cam_heff* h, int* row_slices, int* col_slices, double* workspace_bytes) {
+  CAM_API_BEGIN
+  if (row_slices) *row_slices = h->h.rslices;
+  if (col_slices) *col_slices = h->h.cslices;
+  if (workspace_bytes) *workspace_bytes = h->h.slice_workspace_bytes(h->h.rslices, h->h.cslices);
+  CAM_API_END
+}
+
 int cam_heff_stage_bytes(const cam_heff* h, double* b) {
   CAM_API_BEGIN
   const cam::Heff& H = h->h;
@@ -333,7 +373,7 @@ int cam_heff_get_launch(const cam_heff* h, int precision, int* cfg) {
   CAM_API_BEGIN
   CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
   for (int i = 0; i < 4; ++i) {
-    const LaunchCfg& c = h->h.st[i].cfg[precision];
+    const LaunchCfg& c = h->h.plans[0][i].cfg[precision];
     cfg[4 * i] = c.algo;
     cfg[4 * i + 1] = c.tile_mode;
     cfg[4 * i + 2] = c.wg_per_cu;
@@ -488,15 +528,62 @@ int cam_idmrg_create(cam_ctx* c, const cam_tensor* W, const int16_t* target, int
   CAM_API_END
 }
 
+static void write_record(const cam::IDMRGRecord& r, double* record) {
+  const double v[CAM_IDMRG_RECORD] = {double(r.N),    r.energy,      r.energy_per_vertex, r.entropy,
+                                      r.truncation,   double(r.chi), double(r.n),         r.matvecs_fp64,
+                                      r.matvecs_fp32, r.residual,    r.seconds,           r.heff_flops,
+                                      r.eig_seconds,  r.svd_seconds, r.block_seconds,     r.fidelity,
+                                      r.guess_dropped, double(r.left), double(r.right)};
+  std::copy(v, v + CAM_IDMRG_RECORD, record);
+}
+
 int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record) {
   CAM_API_BEGIN
+  CAM_REQUIRE(d && record, "null argument");
   set_device(d->owner);
-  const cam::IDMRGRecord r = d->d->grow(chi, truncation);
-  const double v[16] = {double(r.N),   r.energy,       r.energy_per_vertex, r.entropy,
-                        r.truncation,  double(r.chi),  double(r.n),         r.matvecs_fp64,
-                        r.matvecs_fp32, r.residual,    r.seconds,           r.heff_flops,
-                        r.eig_seconds, r.svd_seconds,  r.block_seconds,     r.fidelity};
-  std::copy(v, v + 16, record);
+  write_record(d->d->grow(chi, truncation), record);
+  CAM_API_END
+}
+
+int cam_idmrg_move(cam_idmrg* d, int to_right, int64_t chi, double truncation, double* record) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(d && record, "null argument");
+  set_device(d->owner);
+  write_record(d->d->move(to_right != 0, chi, truncation), record);
+  CAM_API_END
+}
+
+int cam_fdmrg_sweeps(cam_idmrg* d, int64_t num_sweeps, int64_t chi, double truncation, double* records,
+                     int32_t* midpoint, int64_t cap, int64_t* nrec) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(d && nrec, "null argument");
+  CAM_REQUIRE(num_sweeps >= 0, "number of sweeps must be >= 0");
+  set_device(d->owner);
+  // a sweep moves right to the end (nright steps), left to the other end
+  // (N - 2) and right to the middle ((N - 2) / 2); every sweep after the
+  // first starts in the middle
+  const int64_t span = d->d->size > 2 ? d->d->size - 2 : 0;
+  *nrec = num_sweeps > 0 && span > 0 ? d->d->nright + span + span / 2 + (num_sweeps - 1) * 2 * span : 0;
+  CAM_REQUIRE(!records || cap >= *nrec, "capacity too small");
+  if (!records) return 0;
+  std::vector<int> mid;
+  const std::vector<cam::IDMRGRecord> rs = d->d->finite_sweeps(num_sweeps, chi, truncation, &mid);
+  CAM_REQUIRE(static_cast<int64_t>(rs.size()) == *nrec, "finite sweeps: unexpected number of steps");
+  for (size_t i = 0; i < rs.size(); ++i) {
+    write_record(rs[i], records + i * CAM_IDMRG_RECORD);
+    if (midpoint) midpoint[i] = mid[i];
+  }
+  CAM_API_END
+}
+
+int cam_idmrg_blocks(const cam_idmrg* d, cam_tensor** LB, cam_tensor** RB) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(d && LB && RB, "null argument");
+  CAM_REQUIRE(d->d->LB && d->d->RB, "iDMRG: no blocks yet (grow first)");
+  auto* l = new cam_tensor{d->d->LB, d->owner};
+  auto* r = new cam_tensor{d->d->RB, d->owner};
+  *LB = l;
+  *RB = r;
   CAM_API_END
 }
 

@@ -1,5 +1,7 @@
 // Library context: one device, one stream, reusable scratch.
 #pragma once
+#include <array>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -23,6 +25,12 @@ struct Ctx {
   // Freeing and re-allocating ~10^11-byte buffers between eigensolves costs
   // seconds near the memory limit, so they are kept until the context ends.
   DevBuf<cplx> ws_krylov, ws_t1, ws_t2;
+  // Leading-values SVD: per block shape (m, n, k), the subspace rounds
+  // before the first residual check (the rounds the last accepted block of
+  // that shape needed; lowered by one after three blocks in a row were
+  // accepted at their first check) and that streak.  It changes only how
+  // many rounds run, never the acceptance test.
+  std::map<std::array<int64_t, 3>, std::array<int, 2>> svd_round_hint;
   explicit Ctx(int dev);
   ~Ctx();
   void* blas_handle();

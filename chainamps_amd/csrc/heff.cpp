@@ -46,7 +46,7 @@ std::map<std::pair<int64_t, int>, EnvMat> env_views(const BSTensor& T, int iout,
 
 }  // namespace
 
-void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
+void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_, double workspace_limit) {
   ctx = &c;
   L = L_;
   W = W_;
@@ -122,11 +122,10 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
       }
   }
 
-  // ---- intermediates ------------------------------------------------------------
+  // ---- intermediates and the block products of the four stages -----------
   t1.clear(); t2.clear(); t3.clear();
+  segA.clear(); segB.clear(); segC.clear(); segD.clear();
   t1n = t2n = t3n = 0;
-  struct SegA { int pb; EnvMat L; };
-  std::map<std::array<int, 5>, SegA> segA;
   for (size_t pi = 0; pi < pblocks.size(); ++pi) {
     const PB& P = pblocks[pi];
     for (int64_t b = 0; b < w; ++b) {
@@ -141,8 +140,6 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     }
   }
   // stage B: T2[b1](s0',s3,s1',s2) <- T1[bL](s0',s3,s1,s2) x W(s1',bL,s1,b1)
-  struct SegW { std::array<int, 5> src; WMat W; };
-  std::map<std::array<int, 5>, std::vector<SegW>> segB, segC;
   for (auto& kv : t1) {
     const auto& k = kv.first;  // (bL, s0', s3, s1, s2)
     for (auto& wv : Wv) {
@@ -177,8 +174,6 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     t3n += tb.m0 * tb.m3 * tb.n1 * tb.n2;
   }
   // stage D: out(s0',s3',s1',s2') <- T3[bR](s0',s3,s1',s2') x R_bR(s3'<-s3)
-  struct SegR { std::array<int, 5> src; EnvMat R; };
-  std::map<int, std::vector<SegR>> segD;   // output psi block -> segments
   for (auto& kv : t3) {
     const auto& k = kv.first;  // (bR, s0', s3, s1', s2')
     auto it = Rv.find({k[0], k[2]});
@@ -188,84 +183,193 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_) {
     CAM_REQUIRE(pit != pindex.end(), "heff: output block outside the allowed space (flux)");
     segD[pit->second].push_back(SegR{k, e});
   }
-
-  bind_workspace();
-
-  // ---- plans: operands addressed as (space, element offset), so the same
-  // plan runs on the FP64 tensors and on their FP32 copies ----------------
-  flops = 0;
-  for (int i = 0; i < 4; ++i) {
-    st[i].clear();
-    stage_flops[i] = 0;
-  }
-  auto off = [](int64_t o) { return reinterpret_cast<const void*>(o); };
-  auto offw = [](int64_t o) { return reinterpret_cast<void*>(o); };
-  for (auto& kv : segA) {
-    const TB& tb = t1.at(kv.first);
-    const PB& Pb = pblocks[kv.second.pb];
-    const EnvMat& e = kv.second.L;
-    const int N = static_cast<int>(Pb.m3 * Pb.n1 * Pb.n2);
-    const int pi = st[0].add_problem(offw(tb.off), tb.m0, static_cast<int>(tb.m0), N, 1, 0, 0, kSpT1);
-    st[0].add_segment(pi, off(e.off), e.ld, 0, off(Pb.off), Pb.m0, 0, static_cast<int>(Pb.m0),
-                      seg_spaces(kSpL, kSpIn));
-  }
-  // W stages: the outputs of every MPO channel and new sector with one row
-  // space read the same operand rows (row group)
-  std::map<std::array<int, 3>, int64_t> rgB, rgC;
-  auto row_group = [](std::map<std::array<int, 3>, int64_t>& m, std::array<int, 3> k) {
-    return m.emplace(k, static_cast<int64_t>(m.size())).first->second;
-  };
-  for (auto& kv : segB) {
-    const TB& tb = t2.at(kv.first);
-    const int64_t rows = tb.m0 * tb.m3;
-    const int pi = st[1].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n1),
-                                     static_cast<int>(tb.n2), rows * tb.n1, 0, kSpT2);
-    st[1].set_group(pi, row_group(rgB, {kv.first[1], kv.first[2], kv.first[4]}));
-    for (const SegW& sg : kv.second) {
-      const TB& sb = t1.at(sg.src);
-      st[1].add_segment(pi, off(sb.off), rows, rows * sb.n1, off(sg.W.off), sg.W.ld, 0,
-                        static_cast<int>(sb.n1), kSegBNContig | seg_spaces(kSpT1, kSpW));
-    }
-  }
-  for (auto& kv : segC) {
-    const TB& tb = t3.at(kv.first);
-    const int64_t rows = tb.m0 * tb.m3 * tb.n1;
-    const int pi = st[2].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n2),
-                                     1, 0, 0, kSpT3);
-    st[2].set_group(pi, row_group(rgC, {kv.first[1], kv.first[2], kv.first[3]}));
-    for (const SegW& sg : kv.second) {
-      const TB& sb = t2.at(sg.src);
-      st[2].add_segment(pi, off(sb.off), rows, 0, off(sg.W.off), sg.W.ld, 0, static_cast<int>(sb.n2),
-                        kSegBNContig | seg_spaces(kSpT2, kSpW));
-    }
-  }
-  for (auto& kv : segD) {
-    const PB& Ob = pblocks[kv.first];
-    const int pi = st[3].add_problem(offw(Ob.off), Ob.m0, static_cast<int>(Ob.m0), static_cast<int>(Ob.m3),
-                                     static_cast<int>(Ob.n1 * Ob.n2), Ob.m0 * Ob.m3, 0, kSpOut);
-    for (const SegR& sg : kv.second) {
-      const TB& sb = t3.at(sg.src);
-      st[3].add_segment(pi, off(sb.off), sb.m0, sb.m0 * sb.m3, off(sg.R.off), sg.R.ld, 0,
-                        static_cast<int>(sb.m3), kSegBNContig | seg_spaces(kSpT3, kSpR));
-    }
-  }
-  for (int i = 0; i < 4; ++i) {
-    st[i].choose_tiles(1);
-    st[i].build_tiles();
-    st[i].upload(ctx->stream);
-    stage_flops[i] = st[i].flops;
-    flops += st[i].flops;
-  }
   // output blocks that receive nothing must be zeroed after the matvec
   untouched.clear();
   for (size_t pi = 0; pi < pblocks.size(); ++pi)
     if (!segD.count(static_cast<int>(pi))) untouched.push_back(static_cast<int>(pi));
+
+  if (workspace_limit < 0.0) {
+    // free device memory plus the context workspaces (reused, grow-only),
+    // less a reserve for the caller's vectors and library scratch
+    size_t fr = 0, tot = 0;
+    HIP_CHECK(hipMemGetInfo(&fr, &tot));
+    const double have = double(fr) + sizeof(cplx) * double(ctx->ws_t1.size() + ctx->ws_t2.size());
+    workspace_limit = have - 4.0e9;
+  }
+  choose_slices(workspace_limit);
+  plan_slices(rslices, cslices);
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
+namespace {
+// rows [lo, hi) of slice i out of k of a sector of m rows
+inline void slice_range(int64_t m, int i, int k, int64_t& lo, int64_t& cnt) {
+  lo = m * i / k;
+  cnt = m * (i + 1) / k - lo;
+}
+}  // namespace
+
+double Heff::slice_workspace_bytes(int rs, int cs) const {
+  // per slice, every intermediate block keeps its (a0' rows, a2 columns)
+  // share; the buffers must hold the largest slice
+  int64_t b1 = 0, b2 = 0;
+  for (int ri = 0; ri < rs; ++ri)
+    for (int ci = 0; ci < cs; ++ci) {
+      auto part = [&](const std::map<std::array<int, 5>, TB>& T) {
+        int64_t tot = 0;
+        for (auto& kv : T) {
+          int64_t r0, mr, c0, mc;
+          slice_range(kv.second.m0, ri, rs, r0, mr);
+          slice_range(kv.second.m3, ci, cs, c0, mc);
+          tot += mr * mc * kv.second.n1 * kv.second.n2;
+        }
+        return tot;
+      };
+      b1 = std::max(b1, std::max(part(t1), part(t3)));
+      b2 = std::max(b2, part(t2));
+    }
+  return double(sizeof(cplx)) * double(b1 + b2);
+}
+
+void Heff::choose_slices(double bytes) {
+  // fewest slices that fit; among equal counts the least extra traffic
+  // (psi read once per row slice, the output read and written once per
+  // column slice after the first)
+  int br = 1, bc = 1;
+  bool found = false;
+  for (int count = 1; count <= 4096 && !found; count *= 2) {
+    double best = 1e300;
+    for (int rs = 1; rs <= count; rs *= 2) {
+      const int cs = count / rs;
+      if (slice_workspace_bytes(rs, cs) > bytes) continue;
+      const double traffic = double(rs) + 2.0 * double(cs - 1);
+      if (traffic < best) {
+        best = traffic;
+        br = rs;
+        bc = cs;
+        found = true;
+      }
+    }
+  }
+  CAM_REQUIRE(found, "heff: the intermediates do not fit the workspace limit even in 4096 slices");
+  rslices = br;
+  cslices = bc;
+}
+
+void Heff::plan_slices(int rs, int cs) {
+  CAM_REQUIRE(rs >= 1 && cs >= 1, "heff: slice counts must be >= 1");
+  rslices = rs;
+  cslices = cs;
+  plans.clear();
+  plans.resize(static_cast<size_t>(rs) * cs);
+  ws1n = ws2n = 0;
+  flops = 0;
+  for (int i = 0; i < 4; ++i) stage_flops[i] = 0;
+  auto off = [](int64_t o) { return reinterpret_cast<const void*>(o); };
+  auto offw = [](int64_t o) { return reinterpret_cast<void*>(o); };
+  for (int ri = 0; ri < rs; ++ri)
+    for (int ci = 0; ci < cs; ++ci) {
+      std::array<GemmPlan, 4>& st = plans[static_cast<size_t>(ri) * cs + ci];
+      // slice-local layouts of the intermediates: block (b, s0', s3, sx, sy)
+      // keeps rows [r0, r0+mr) of its a0' sector and columns [c0, c0+mc) of
+      // its a2 sector, dense over (a0', a2, sx, sy)
+      struct SB { int64_t off, r0, mr, c0, mc, n1, n2; };
+      auto layout = [&](const std::map<std::array<int, 5>, TB>& T, std::map<std::array<int, 5>, SB>& out) {
+        int64_t tot = 0;
+        for (auto& kv : T) {
+          SB b{tot, 0, 0, 0, 0, kv.second.n1, kv.second.n2};
+          slice_range(kv.second.m0, ri, rs, b.r0, b.mr);
+          slice_range(kv.second.m3, ci, cs, b.c0, b.mc);
+          out[kv.first] = b;
+          tot += b.mr * b.mc * b.n1 * b.n2;
+        }
+        return tot;
+      };
+      std::map<std::array<int, 5>, SB> l1, l2, l3;
+      const int64_t n1 = layout(t1, l1), n2 = layout(t2, l2), n3 = layout(t3, l3);
+      ws1n = std::max(ws1n, std::max(n1, n3));
+      ws2n = std::max(ws2n, n2);
+      // stage A: T1 rows of the slice from L_b's rows, psi's columns of the slice
+      for (auto& kv : segA) {
+        const SB& tb = l1.at(kv.first);
+        if (!tb.mr || !tb.mc) continue;
+        const PB& Pb = pblocks[kv.second.pb];
+        const EnvMat& e = kv.second.L;
+        if (tb.mc == Pb.m3) {
+          const int N = static_cast<int>(Pb.m3 * Pb.n1 * Pb.n2);
+          const int pi = st[0].add_problem(offw(tb.off), tb.mr, static_cast<int>(tb.mr), N, 1, 0, 0, kSpT1);
+          st[0].add_segment(pi, off(e.off + tb.r0), e.ld, 0, off(Pb.off), Pb.m0, 0, static_cast<int>(Pb.m0),
+                            seg_spaces(kSpL, kSpIn));
+        } else {
+          // a2 columns of one (s1, s2) slab are contiguous: batch over the slabs
+          const int pi = st[0].add_problem(offw(tb.off), tb.mr, static_cast<int>(tb.mr), static_cast<int>(tb.mc),
+                                           static_cast<int>(Pb.n1 * Pb.n2), tb.mr * tb.mc, 0, kSpT1);
+          st[0].add_segment(pi, off(e.off + tb.r0), e.ld, 0, off(Pb.off + Pb.m0 * tb.c0), Pb.m0, Pb.m0 * Pb.m3,
+                            static_cast<int>(Pb.m0), seg_spaces(kSpL, kSpIn));
+        }
+      }
+      // W stages: the outputs of every MPO channel and new sector with one
+      // row space read the same operand rows (row group)
+      std::map<std::array<int, 3>, int64_t> rgB, rgC;
+      auto row_group = [](std::map<std::array<int, 3>, int64_t>& m, std::array<int, 3> k) {
+        return m.emplace(k, static_cast<int64_t>(m.size())).first->second;
+      };
+      for (auto& kv : segB) {
+        const SB& tb = l2.at(kv.first);
+        const int64_t rows = tb.mr * tb.mc;
+        if (!rows) continue;
+        const int pi = st[1].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n1),
+                                         static_cast<int>(tb.n2), rows * tb.n1, 0, kSpT2);
+        st[1].set_group(pi, row_group(rgB, {kv.first[1], kv.first[2], kv.first[4]}));
+        for (const SegW& sg : kv.second) {
+          const SB& sb = l1.at(sg.src);
+          st[1].add_segment(pi, off(sb.off), rows, rows * sb.n1, off(sg.W.off), sg.W.ld, 0,
+                            static_cast<int>(sb.n1), kSegBNContig | seg_spaces(kSpT1, kSpW));
+        }
+      }
+      for (auto& kv : segC) {
+        const SB& tb = l3.at(kv.first);
+        const int64_t rows = tb.mr * tb.mc * tb.n1;
+        if (!rows) continue;
+        const int pi = st[2].add_problem(offw(tb.off), rows, static_cast<int>(rows), static_cast<int>(tb.n2), 1,
+                                         0, 0, kSpT3);
+        st[2].set_group(pi, row_group(rgC, {kv.first[1], kv.first[2], kv.first[3]}));
+        for (const SegW& sg : kv.second) {
+          const SB& sb = l2.at(sg.src);
+          st[2].add_segment(pi, off(sb.off), rows, 0, off(sg.W.off), sg.W.ld, 0, static_cast<int>(sb.n2),
+                            kSegBNContig | seg_spaces(kSpT2, kSpW));
+        }
+      }
+      // stage D: rows of the slice of every output block; the first column
+      // slice writes, the others accumulate their part of the a2 sum
+      for (auto& kv : segD) {
+        const PB& Ob = pblocks[kv.first];
+        int64_t r0, mr;
+        slice_range(Ob.m0, ri, rs, r0, mr);
+        if (!mr) continue;
+        const int pi = st[3].add_problem(offw(Ob.off + r0), Ob.m0, static_cast<int>(mr), static_cast<int>(Ob.m3),
+                                         static_cast<int>(Ob.n1 * Ob.n2), Ob.m0 * Ob.m3, ci > 0 ? 1 : 0, kSpOut);
+        for (const SegR& sg : kv.second) {
+          const SB& sb = l3.at(sg.src);
+          if (!sb.mc) continue;
+          st[3].add_segment(pi, off(sb.off), sb.mr, sb.mr * sb.mc, off(sg.R.off + sg.R.ld * sb.c0), sg.R.ld, 0,
+                            static_cast<int>(sb.mc), kSegBNContig | seg_spaces(kSpT3, kSpR));
+        }
+      }
+      for (int i = 0; i < 4; ++i) {
+        st[i].choose_tiles(1);
+        st[i].build_tiles();
+        st[i].upload(ctx->stream);
+        stage_flops[i] += st[i].flops;
+        flops += st[i].flops;
+      }
+    }
+  bind_workspace();
+}
+
 void Heff::bind_workspace() {
-  ctx->ws_t1.ensure(std::max<int64_t>(std::max(t1n, t3n), 1));
-  ctx->ws_t2.ensure(std::max<int64_t>(t2n, 1));
+  ctx->ws_t1.ensure(std::max<int64_t>(ws1n, 1));
+  ctx->ws_t2.ensure(std::max<int64_t>(ws2n, 1));
   T1.p = ctx->ws_t1.get();
   T2.p = ctx->ws_t2.get();
 }
@@ -303,8 +407,6 @@ void Heff::prepare_fp32() {
   launch_to_fp32(L->data.get(), L32.get(), L->total, s);
   launch_to_fp32(W->data.get(), W32.get(), W->total, s);
   launch_to_fp32(R->data.get(), R32.get(), R->total, s);
-  vinf.alloc(std::max<int64_t>(n, 1));
-  voutf.alloc(std::max<int64_t>(n, 1));
   HIP_CHECK(hipStreamSynchronize(s));
   fp32_ready = true;
 }
@@ -317,30 +419,42 @@ bool stream_capturing(hipStream_t s) {
 }  // namespace
 
 // The first matvec at a precision (outside graph capture) autotunes every
-// stage plan on its real operands: stage i is re-run while stage i-1's
-// output is already in place, and the last run leaves the correct result.
+// stage plan of the first slice on its real operands: stage i is re-run
+// while stage i-1's output is already in place, and the last run leaves
+// the correct result (the first slice's stage D writes, it never
+// accumulates).  The other slices have the same shapes up to a row or
+// column and take the first slice's configuration.
 void Heff::run_stages(int precision, const GemmBases& b) {
   hipStream_t s = ctx->stream;
   const bool tune = !tuned(precision) && !stream_capturing(s);
-  for (int i = 0; i < 4; ++i) {
-    if (tune) autotune(st[i], precision, s, &b);
-    launch_grouped_gemm(st[i], precision, s, &b);
+  for (size_t k = 0; k < plans.size(); ++k) {
+    std::array<GemmPlan, 4>& st = plans[k];
+    for (int i = 0; i < 4; ++i) {
+      if (tune && k == 0) autotune(st[i], precision, s, &b);
+      if (tune && k > 0 && plans[0][i].tuned[precision]) {
+        st[i].cfg[precision] = plans[0][i].cfg[precision];
+        st[i].tuned[precision] = true;
+      }
+      launch_grouped_gemm(st[i], precision, s, &b);
+    }
   }
 }
 
 bool Heff::tuned(int precision) const {
-  for (int i = 0; i < 4; ++i)
-    if (!st[i].tuned[precision] && st[i].flops >= 2e10) return false;
+  for (const auto& st : plans)
+    for (int i = 0; i < 4; ++i)
+      if (!st[i].tuned[precision] && plans[0][i].flops >= 2e10) return false;
   return true;
 }
 
 void Heff::set_launch(int precision, const LaunchCfg& c) {
-  for (int i = 0; i < 4; ++i) {
-    st[i].cfg[precision] = c;
-    st[i].tuned[precision] = true;
-    if (!st[i].built[c.tile_mode]) st[i].build_tiles_mode(c.tile_mode);
-    st[i].upload(ctx->stream);
-  }
+  for (auto& st : plans)
+    for (int i = 0; i < 4; ++i) {
+      st[i].cfg[precision] = c;
+      st[i].tuned[precision] = true;
+      if (!st[i].built[c.tile_mode]) st[i].build_tiles_mode(c.tile_mode);
+      st[i].upload(ctx->stream);
+    }
 }
 
 void Heff::apply_blocked(const cplx* in, cplx* out) {
@@ -391,8 +505,9 @@ void Heff::add_projectors32(const float2* in, float2* out) {
 }
 
 void Heff::forward_stages(const cplx* in, int nstages) {
+  CAM_REQUIRE(nslices() == 1, "heff: the adjoint needs the whole intermediates (an unsliced matvec)");
   const GemmBases b = bases(0, in, nullptr);
-  for (int i = 0; i < std::min(nstages, 3); ++i) launch_grouped_gemm(st[i], 0, ctx->stream, &b);
+  for (int i = 0; i < std::min(nstages, 3); ++i) launch_grouped_gemm(plans[0][i], 0, ctx->stream, &b);
 }
 
 void Heff::apply_blocked32(const float2* in, float2* out) {
@@ -410,6 +525,8 @@ void Heff::apply(const cplx* in, cplx* out, int precision) {
   }
   prepare_fp32();
   hipStream_t s = ctx->stream;
+  vinf.ensure(std::max<int64_t>(n, 1));
+  voutf.ensure(std::max<int64_t>(n, 1));
   launch_to_fp32(in, vinf.get(), n, s);
   apply_blocked32(vinf.get(), voutf.get());
   launch_to_fp64(voutf.get(), out, n, s);
@@ -478,42 +595,52 @@ void Heff::vector_from_tensor(const BSTensor& T, cplx* x) {
   launch_permute(jobs, ctx->perm_jobs, ctx->perm_chunks, ctx->stream);
 }
 
-void Heff::profile(int precision, int reps, double* stage_ms) {
+void Heff::profile(int precision, int reps, double* stage_ms, const void* in_dev, void* out_dev) {
   hipStream_t s = ctx->stream;
   if (precision == 1) prepare_fp32();
-  DevBuf<cplx> x(std::max<int64_t>(n, 1)), y(std::max<int64_t>(n, 1));
+  DevBuf<cplx> x, y;
   DevBuf<float2> xf, yf;
-  // deterministic non-zero input
-  std::vector<cplx> h(static_cast<size_t>(std::min<int64_t>(n, 1 << 20)));
-  for (size_t i = 0; i < h.size(); ++i) h[i] = cplx{std::sin(0.37 * i), std::cos(0.11 * i)};
-  for (int64_t o = 0; o < n; o += static_cast<int64_t>(h.size()))
-    HIP_CHECK(hipMemcpyAsync(x.get() + o, h.data(), std::min<int64_t>(h.size(), n - o) * sizeof(cplx),
-                             hipMemcpyHostToDevice, s));
-  const void* in = x.get();
-  void* out = y.get();
-  if (precision == 1) {
-    xf.alloc(std::max<int64_t>(n, 1));
-    yf.alloc(std::max<int64_t>(n, 1));
-    launch_to_fp32(x.get(), xf.get(), n, s);
-    in = xf.get();
-    out = yf.get();
+  const void* in = in_dev;
+  void* out = out_dev;
+  if (!in || !out) {
+    x.alloc(std::max<int64_t>(n, 1));
+    y.alloc(std::max<int64_t>(n, 1));
+    // deterministic non-zero input
+    std::vector<cplx> h(static_cast<size_t>(std::min<int64_t>(n, 1 << 20)));
+    for (size_t i = 0; i < h.size(); ++i) h[i] = cplx{std::sin(0.37 * i), std::cos(0.11 * i)};
+    for (int64_t o = 0; o < n; o += static_cast<int64_t>(h.size()))
+      HIP_CHECK(hipMemcpyAsync(x.get() + o, h.data(), std::min<int64_t>(h.size(), n - o) * sizeof(cplx),
+                               hipMemcpyHostToDevice, s));
+    in = x.get();
+    out = y.get();
+    if (precision == 1) {
+      xf.alloc(std::max<int64_t>(n, 1));
+      yf.alloc(std::max<int64_t>(n, 1));
+      launch_to_fp32(x.get(), xf.get(), n, s);
+      in = xf.get();
+      out = yf.get();
+    }
   }
   const GemmBases b = bases(precision, in, out);
-  hipEvent_t ev[5];
+  // one event before the matvec and one after every (slice, stage) launch;
+  // a stage's time is the sum over the slices of its launches
+  const size_t nl = 4 * plans.size();
+  std::vector<hipEvent_t> ev(nl + 1);
   for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
   double acc[4] = {0, 0, 0, 0};
   for (int r = 0; r < reps + 1; ++r) {            // first pass is a warmup
     HIP_CHECK(hipEventRecord(ev[0], s));
-    for (int i = 0; i < 4; ++i) {
-      launch_grouped_gemm(st[i], precision, s, &b);
-      HIP_CHECK(hipEventRecord(ev[i + 1], s));
-    }
-    HIP_CHECK(hipEventSynchronize(ev[4]));
+    for (size_t k = 0; k < plans.size(); ++k)
+      for (int i = 0; i < 4; ++i) {
+        launch_grouped_gemm(plans[k][i], precision, s, &b);
+        HIP_CHECK(hipEventRecord(ev[4 * k + i + 1], s));
+      }
+    HIP_CHECK(hipEventSynchronize(ev[nl]));
     if (r == 0) continue;
-    for (int i = 0; i < 4; ++i) {
+    for (size_t j = 0; j < nl; ++j) {
       float ms = 0;
-      HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-      acc[i] += ms;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[j], ev[j + 1]));
+      acc[j % 4] += ms;
     }
   }
   for (auto& e : ev) HIP_CHECK(hipEventDestroy(e));

@@ -50,10 +50,29 @@ struct Heff {
   // W (bl, br, s', s) -> view
   std::map<std::pair<int64_t, int>, EnvMat> Lv, Rv;
   std::map<std::array<int64_t, 4>, WMat> Wv;
-  // intermediates: key (b, s0', s3, sx, sy) -> offset; sizes
+  // intermediates of the whole matvec: key (b, s0', s3, sx, sy) -> offset;
+  // sizes (the layout the adjoint works on)
   struct TB { int64_t off, m0, m3, n1, n2; };
   std::map<std::array<int, 5>, TB> t1, t2, t3;
   int64_t t1n = 0, t2n = 0, t3n = 0;
+  // the block products of the four stages (independent of slicing)
+  struct SegA { int pb; EnvMat L; };
+  struct SegW { std::array<int, 5> src; WMat W; };
+  struct SegR { std::array<int, 5> src; EnvMat R; };
+  std::map<std::array<int, 5>, SegA> segA;
+  std::map<std::array<int, 5>, std::vector<SegW>> segB, segC;
+  std::map<int, std::vector<SegR>> segD;
+
+  // Slicing: the a0' rows of every sector are cut into `rslices` ranges and
+  // the a2 columns of every sector into `cslices`; slice (r, c) runs all four
+  // stages on intermediates restricted to those rows and columns (the
+  // stages act on a0' and a2 independently until stage D contracts a2, which
+  // accumulates the column slices into the output).  Intermediates then
+  // take 1/(rslices cslices) of the memory, at the cost of reading psi
+  // rslices times and the output cslices times.  1 x 1 is the plain matvec.
+  int rslices = 1, cslices = 1;
+  int64_t ws1n = 0, ws2n = 0;                  // workspace elements (max over slices)
+  std::vector<std::array<GemmPlan, 4>> plans;  // per slice (r-major), stages A..D
 
   // Two intermediate buffers: stage C writes T3 over T1 (stage D reads only
   // T3), and an FP32 matvec uses the same bytes as float2 (half the size),
@@ -62,7 +81,6 @@ struct Heff {
   DevBuf<cplx> vin, vout;
   DevBuf<float2> L32, W32, R32, vinf, voutf;   // FP32 operands
   bool fp32_ready = false;
-  GemmPlan st[4];
   std::vector<int> untouched;   // psi blocks no stage-D problem writes
   double flops = 0.0;
   double stage_flops[4] = {0, 0, 0, 0};
@@ -72,7 +90,17 @@ struct Heff {
   DevBuf<int64_t> perm;           // perm[i] = block-layout offset of allowed[i]
   bool perm_ready = false;
 
-  void build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_);
+  // workspace_limit: bytes the two intermediate buffers may take (< 0:
+  // the device's free memory plus what the context workspaces already
+  // hold, less a reserve); the fewest slices that fit are used
+  void build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_, double workspace_limit = -1.0);
+  // (re)plans the stages for rs x cs slices and binds the workspaces
+  void plan_slices(int rs, int cs);
+  // bytes of the two intermediate buffers for rs x cs slices
+  double slice_workspace_bytes(int rs, int cs) const;
+  // the fewest slices (r x c, powers of two) whose workspace fits `bytes`
+  void choose_slices(double bytes);
+  int nslices() const { return rslices * cslices; }
   void build_perm();
   void prepare_fp32();
   // operand bases of a launch (re-binds T1/T2 to the context workspaces)
@@ -95,8 +123,10 @@ struct Heff {
   TensorPtr tensor_from_vector(const cplx* x);
   // block-layout vector from such a tensor; blocks it lacks are zero
   void vector_from_tensor(const BSTensor& T, cplx* x);
-  // average ms per stage over `reps` matvecs (HIP events on the stream)
-  void profile(int precision, int reps, double* stage_ms);
+  // average ms per stage over `reps` matvecs (HIP events on the stream),
+  // on the given device block-layout vectors (FP64, or FP32 split complex
+  // for precision 1) or, when null, on internal ones
+  void profile(int precision, int reps, double* stage_ms, const void* in_dev = nullptr, void* out_dev = nullptr);
 
   // Excited-state projector terms of TwoVertexMPOMPSMultiply
   // (DMRG_routines.cpp:947-973): out += weight_p conj(P_p) (P_p . in), the

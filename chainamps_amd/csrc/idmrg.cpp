@@ -12,9 +12,15 @@
 //   step k: MakeLeftBlock/MakeRightBlock from the previous U / V^dagger
 //           (skipped at size 2, whose blocks were made at step 0),
 //           TwoVertexWavefunction with the predicted guess, TwoVertexSVD,
-//           SquareRescale(1), MakePrediction (DMRG_routines.cpp:693-707).
-// Every tensor stays on the device; the disk block store of the reference
-// (BlocksStructure, DMRG_routines.cpp:57-186) is not reproduced.
+//           SquareRescale(1), MakePrediction (DMRG_routines.cpp:693-707);
+//   finite sweeps: SuperBlock::move_right_two_vertex / move_left_two_vertex
+//           (DMRG_routines.cpp:376-444) with MakeRFinitePrediction /
+//           MakeLFinitePrediction (709-716) and the finite CheckConvergence
+//           (773-776), driven by FiniteDMRG::run (fDMRG_routines.cpp:19-66).
+// Every tensor stays on the device: the reference's disk block store
+// (BlocksStructure, DMRG_routines.cpp:57-186: one .BLOCK file per block
+// size, one .MPS_matrix file per vertex) is a set of device-resident
+// tensors indexed the same way.
 #include <chrono>
 #include <cmath>
 
@@ -123,6 +129,23 @@ TensorPtr make_prediction(Ctx& ctx, const BSTensor& U, const std::vector<double>
   return contract(ctx, *ALambdaR, false, *LambdaLB, false, {{2, 0}});
 }
 
+TensorPtr make_l_finite_prediction(Ctx& ctx, const BSTensor& U, const std::vector<double>& values,
+                                   const BSTensor& A) {
+  // A . (U . diag(values)): A's right bond against U's left bond
+  TensorPtr Us = copy_tensor(ctx, U);
+  scale_along_index(ctx, *Us, 2, values);
+  return contract(ctx, A, false, *Us, false, {{2, 1}});
+}
+
+TensorPtr make_r_finite_prediction(Ctx& ctx, const std::vector<double>& values, const BSTensor& Vd,
+                                   const BSTensor& B) {
+  // reorder(diag(values) . Vd, 102, 2) . B: Vd's right bond against B's left bond
+  TensorPtr Vs = copy_tensor(ctx, Vd);
+  scale_along_index(ctx, *Vs, 0, values);
+  TensorPtr T = reorder(ctx, *Vs, false, {1, 0, 2}, 2);
+  return contract(ctx, *T, false, B, false, {{2, 0}});
+}
+
 double fidelity(Ctx& ctx, const std::vector<double>& values, const BSTensor& Vd, const std::vector<double>& prev) {
   // MakePrediction's Auxiliary = diag(s') W^dagger comes from the SVD
   // ALambdaR = reorder(diag(values) . Vd, 102, 2) = C s' W^dagger, so
@@ -144,9 +167,19 @@ double fidelity(Ctx& ctx, const std::vector<double>& values, const BSTensor& Vd,
 IDMRG::IDMRG(Ctx& c, TensorPtr mpo, const Charge& tgt, bool conv, bool mix, int k)
     : ctx(&c), W(std::move(mpo)), target(tgt), converge(conv), mixed(mix), ncv(k) {
   CAM_REQUIRE(W->nidx() == 4, "iDMRG needs a 4-index MPO");
+  CAM_REQUIRE(ncv >= 2 && ncv <= 63, "iDMRG: ncv must be 2..63");
 }
 
+namespace {
+template <typename T>
+void set_at(std::vector<T>& v, int64_t k, T x) {
+  if (static_cast<int64_t>(v.size()) <= k) v.resize(k + 1);
+  v[k] = std::move(x);
+}
+}  // namespace
+
 IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
+  CAM_REQUIRE(!swept, "iDMRG: growth after finite sweeps is not supported (FiniteDMRG does not grow)");
   const auto t0 = std::chrono::steady_clock::now();
   // phase clock: the stream is drained so each phase is charged its own work
   auto lap = [this](std::chrono::steady_clock::time_point& t) {
@@ -179,12 +212,20 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
     RB = make_right_block(*ctx, *RBd, *W, *Vd);
     size = 2;
     prev_lambda = {1.0};
+    lblk = {LBd, LB};
+    rblk = {RBd, RB};
+    lmps = {nullptr, U};
+    rmps = {nullptr, Vd};
+    nleft = nright = 0;
   } else {
     if (size > 2) {
       LB = make_left_block(*ctx, *LB, *W, *U);
       RB = make_right_block(*ctx, *RB, *W, *Vd);
+      set_at(lblk, size / 2, LB);
+      set_at(rblk, size / 2, RB);
     }
     size += 2;
+    nleft = nright = size / 2 - 1;
     prev_lambda = values;
     H.build(*ctx, LB, W, RB);
     xbuf.ensure(std::max<int64_t>(H.n, 1));
@@ -201,19 +242,24 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
     svd_truncate(*ctx, *psi, chi, trunc, U, Vd, values, disc);
     truncation = disc;
     rec.svd_seconds = lap(tp);
+    set_at(lmps, size / 2, U);
+    set_at(rmps, size / 2, Vd);
   }
-  // SquareRescale(1.0) (MPS_matrix.hpp:66-70): Truncation/1, values to unit square sum
-  double sq = 0.0;
-  for (double v : values) sq += v * v;
-  CAM_REQUIRE(sq > 0.0, "Square sum error");
-  for (double& v : values) v *= std::sqrt(1.0 / sq);
-  try {
+  square_rescale(values);
+  // MakePrediction maps Vd's right bond onto U's left bond; when the two
+  // carry different charges (a non-zero target) there is no prediction and
+  // the next solve starts from a random vector, which the record reports.
+  // Every other failure propagates.
+  if (Vd->idx[2].same_charges(U->idx[1])) {
     guess = make_prediction(*ctx, *U, values, *Vd, prev_lambda);
-  } catch (const Error&) {
-    guess.reset();   // inconsistent bond charges: next solve starts at random
+    guess_dropped = false;
+  } else {
+    guess.reset();
+    guess_dropped = true;
   }
   // 1 - fidelity of the prediction (CheckConvergence, DMRG_routines.cpp:769-771)
   rec.fidelity = fidelity(*ctx, values, *Vd, prev_lambda);
+  rec.guess_dropped = guess_dropped ? 1.0 : 0.0;
   rec.block_seconds += lap(tp);
   rec.N = size;
   rec.energy = E;
@@ -227,7 +273,122 @@ IDMRGRecord IDMRG::grow(int64_t chi, double trunc) {
   rec.residual = info[3];
   rec.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   rec.heff_flops = H.flops;
+  rec.left = nleft;
+  rec.right = nright;
   return rec;
+}
+
+// SquareRescale(1.0) (MPS_matrix.hpp:66-70): values to unit square sum
+void IDMRG::square_rescale(std::vector<double>& v) {
+  double sq = 0.0;
+  for (double x : v) sq += x * x;
+  CAM_REQUIRE(sq > 0.0, "Square sum error");
+  for (double& x : v) x *= std::sqrt(1.0 / sq);
+}
+
+IDMRGRecord IDMRG::move(bool to_right, int64_t chi, double trunc) {
+  CAM_REQUIRE(size >= 4, "finite sweeps need at least four vertices (two are stored as they are)");
+  const auto t0 = std::chrono::steady_clock::now();
+  auto lap = [this](std::chrono::steady_clock::time_point& t) {
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto now = std::chrono::steady_clock::now();
+    const double dt = std::chrono::duration<double>(now - t).count();
+    t = now;
+    return dt;
+  };
+  auto tp = t0;
+  IDMRGRecord rec{};
+  // shift the centre: the vertex that leaves it joins the block on the side
+  // it leaves towards, the stored MPS matrix of the vertex that enters it
+  // gives the prediction (the truncation is switched off next to the ends)
+  TensorPtr pred;
+  double t = trunc;
+  if (to_right) {
+    CAM_REQUIRE(nright >= 1, "At right end of superblock already...");
+    TensorPtr nl = make_left_block(*ctx, *lblk[nleft], *W, *U);
+    ++nleft;
+    --nright;
+    set_at(lblk, nleft, nl);
+    CAM_REQUIRE(static_cast<int64_t>(rmps.size()) > nright + 1 && rmps[nright + 1], "finite DMRG: no stored right matrix");
+    pred = make_r_finite_prediction(*ctx, values, *Vd, *rmps[nright + 1]);
+    if (nright <= 1) t = -0.0;
+  } else {
+    CAM_REQUIRE(nleft >= 1, "At left end of superblock already...");
+    TensorPtr nr = make_right_block(*ctx, *rblk[nright], *W, *Vd);
+    ++nright;
+    --nleft;
+    set_at(rblk, nright, nr);
+    CAM_REQUIRE(static_cast<int64_t>(lmps.size()) > nleft + 1 && lmps[nleft + 1], "finite DMRG: no stored left matrix");
+    pred = make_l_finite_prediction(*ctx, *U, values, *lmps[nleft + 1]);
+    if (nleft <= 1) t = -0.0;
+  }
+  swept = true;
+  LB = lblk[nleft];
+  RB = rblk[nright];
+  prev_lambda = values;
+  Heff H;
+  H.build(*ctx, LB, W, RB);
+  xbuf.ensure(std::max<int64_t>(H.n, 1));
+  gbuf.ensure(std::max<int64_t>(H.n, 1));
+  H.vector_from_tensor(*pred, gbuf.get());
+  launch_copy(gbuf.get(), xbuf.get(), H.n, ctx->stream);
+  rec.block_seconds += lap(tp);
+  double info[4] = {0, 0, 0, 0};
+  const double E = eigs_lowest(H, xbuf.get(), true, converge, 0.0, ncv, mixed, info);
+  rec.eig_seconds = lap(tp);
+  TensorPtr psi = H.tensor_from_vector(xbuf.get());
+  double disc = 0.0;
+  svd_truncate(*ctx, *psi, chi, t, U, Vd, values, disc);
+  truncation = disc;
+  square_rescale(values);
+  set_at(lmps, nleft + 1, U);
+  set_at(rmps, nright + 1, Vd);
+  rec.svd_seconds = lap(tp);
+  // CheckConvergence(Prediction, MPSDecomposition): 1 - |<guess|U s Vd>|
+  TensorPtr Us = copy_tensor(*ctx, *U);
+  scale_along_index(*ctx, *Us, 2, values);
+  TensorPtr now = contract(*ctx, *Us, false, *Vd, false, {{2, 0}});
+  H.vector_from_tensor(*now, xbuf.get());
+  DevBuf<cplx> part(static_cast<size_t>(multi_dot_blocks(H.n))), c(1);
+  launch_multi_dot(gbuf.get(), H.n, 1, xbuf.get(), H.n, part.get(), c.get(), ctx->stream);
+  cplx ov{0, 0};
+  HIP_CHECK(hipMemcpyAsync(&ov, c.get(), sizeof(cplx), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  rec.fidelity = 1.0 - std::hypot(ov.re, ov.im);
+  guess.reset();
+  rec.block_seconds += lap(tp);
+  rec.N = size;
+  rec.energy = E;
+  rec.energy_per_vertex = E / static_cast<double>(size);
+  rec.entropy = entropy(values);
+  rec.truncation = truncation;
+  rec.chi = static_cast<int64_t>(values.size());
+  rec.n = H.n;
+  rec.matvecs_fp64 = info[0];
+  rec.matvecs_fp32 = info[1];
+  rec.residual = info[3];
+  rec.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  rec.heff_flops = H.flops;
+  rec.left = nleft;
+  rec.right = nright;
+  return rec;
+}
+
+std::vector<IDMRGRecord> IDMRG::finite_sweeps(int64_t num_sweeps, int64_t chi, double trunc,
+                                               std::vector<int>* midpoint) {
+  std::vector<IDMRGRecord> out;
+  if (size <= 2) return out;   // two vertices: nothing to sweep
+  auto push = [&](const IDMRGRecord& r, bool record_mid) {
+    out.push_back(r);
+    if (midpoint) midpoint->push_back(record_mid && nleft == nright ? 1 : 0);
+  };
+  for (int64_t sweep = 0; sweep < num_sweeps; ++sweep) {
+    for (int64_t r = nright; r > 0; --r) push(move(true, chi, trunc), false);
+    for (int64_t l = nleft; l > 0; --l) push(move(false, chi, trunc), true);
+    // for (r = right_size(); r > left_size(); --r): both sides move
+    for (int64_t r = nright; r > nleft; --r) push(move(true, chi, trunc), true);
+  }
+  return out;
 }
 
 }  // namespace cam

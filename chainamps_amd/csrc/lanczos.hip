@@ -515,6 +515,7 @@ double dense_lowest(Heff& H, cplx* xout, double* info) {
 
 double eigs_lowest(Heff& H, cplx* x, bool have_start, bool converge, double tol, int ncv, bool mixed,
                    double* info) {
+  CAM_REQUIRE(ncv <= 0 || (ncv >= 2 && ncv <= 63), "eigensolver: ncv must be 2..63 (or <= 0 for 20)");
   // the dense route only without projector terms (DMRG_routines.cpp:852)
   if (H.n <= kSparseThreshold && H.nproj == 0) return dense_lowest(H, x, info);
   return lanczos_lowest(H, x, have_start, converge, tol, ncv, mixed, info);

@@ -179,17 +179,11 @@ void orthonormalise(Ctx& ctx, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& G, De
   householder_orthonormalise(static_cast<rocblas_handle>(ctx.blas_handle()), X, m, k, tau);
 }
 
-// Rounds the last accepted decomposition of each (device, m, n, k) needed:
-// the first residual check (a Rayleigh-Ritz projection, a k x k core and
-// the residuals, about 1.5 rounds' worth) waits until then, since the
-// blocks of consecutive iDMRG / iTEBD steps converge alike.
-std::mutex g_rounds_mu;
-std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> g_rounds;
-
 }  // namespace
 
 bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int64_t ncheck, double tol,
-                   int max_rounds, std::vector<double>& S, DevBuf<cplx>& U, DevBuf<cplx>& Vh, int* rounds) {
+                   int max_rounds, std::vector<double>& S, DevBuf<cplx>& U, DevBuf<cplx>& Vh, int* rounds,
+                   double* last_res) {
   CAM_REQUIRE(k >= 1 && k <= std::min(m, n), "truncated_svd: k out of range");
   CAM_REQUIRE(m < (int64_t(1) << 31) && n < (int64_t(1) << 31), "truncated_svd: block too large");
   hipStream_t s = ctx.stream;
@@ -202,14 +196,17 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   launch_random_fill(W.get(), n * k, 0x5eed0000ull + static_cast<uint64_t>(m * 31 + n), s);
   gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
   int q = 0;
-  int dev = 0;
-  HIP_CHECK(hipGetDevice(&dev));
-  const auto shape = std::make_tuple(dev, m, n, k);
-  int first = (m * n > (int64_t(1) << 24)) ? 4 : 2;
+  // The first residual check (a Rayleigh-Ritz projection, a k x k core and
+  // the residuals, about 1.5 rounds' worth) waits for the rounds the last
+  // accepted block of this shape on this context needed, since the blocks
+  // of consecutive iDMRG / iTEBD steps converge alike (the hint only moves
+  // the check; acceptance is always the residual test below).
+  const std::array<int64_t, 3> shape{m, n, k};
+  const int first_default = (m * n > (int64_t(1) << 24)) ? 4 : 2;
+  int first = first_default;
   {
-    std::lock_guard<std::mutex> lk(g_rounds_mu);
-    auto it = g_rounds.find(shape);
-    if (it != g_rounds.end()) first = std::max(first, std::min(it->second, max_rounds));
+    auto it = ctx.svd_round_hint.find(shape);
+    if (it != ctx.svd_round_hint.end()) first = std::max(first, std::min(it->second[0], max_rounds));
   }
   // CAM_SVD_TRACE=1: time of the subspace rounds, the Rayleigh-Ritz
   // projection, the k x k core and the residual check of every check
@@ -280,8 +277,37 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
                    t_round, t_proj, t_core, t_res);
     if (rounds) *rounds = q;
     if (worst <= tol) {
-      std::lock_guard<std::mutex> lk(g_rounds_mu);
-      g_rounds[shape] = q;
+      if (last_res) {
+        // residual of the last computed triplet (the slowest to converge):
+        // bounds how far its value may sit below a singular value of A
+        *last_res = 0.0;
+        if (ncheck < k) {
+          gemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose, m, 1, n, A, m, Vh.get() + (k - 1),
+               k, E.get(), m);
+          hipLaunchKernelGGL(ritz_residual_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<double2*>(E.get()), m,
+                             reinterpret_cast<const double2*>(U.get() + (k - 1) * m), dS.get() + (k - 1),
+                             dres.get());
+          HIP_CHECK(hipGetLastError());
+          double r2 = 0.0;
+          HIP_CHECK(hipMemcpyAsync(&r2, dres.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+          HIP_CHECK(hipStreamSynchronize(s));
+          *last_res = std::sqrt(r2);
+        }
+      }
+      // accepted at the first check: after three such blocks in a row the
+      // hint is lowered by one round; otherwise it becomes the rounds needed
+      auto& h = ctx.svd_round_hint[shape];
+      if (q == first && q > first_default) {
+        if (++h[1] >= 3) {
+          h[0] = q - 1;
+          h[1] = 0;
+        } else {
+          h[0] = q;
+        }
+      } else {
+        h[0] = q;
+        h[1] = 0;
+      }
       return true;
     }
     if (q >= max_rounds) return false;

@@ -131,6 +131,7 @@ struct Group {
   std::vector<double> S;
   bool truncated = false;                        // only the leading k values computed
   double tail = 0.0;                             // weight - sum S^2 when truncated
+  double last_res = 0.0;                         // residual of the last computed triplet
 };
 
 }  // namespace
@@ -330,7 +331,7 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
       int rounds = 0;
       const auto t0 = std::chrono::steady_clock::now();
       const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, std::min(D, kk), kRitzTol * std::sqrt(g.weight),
-                                    24, S, g.U, g.Vh, &rounds);
+                                    24, S, g.U, g.Vh, &rounds, &g.last_res);
       if (trace)
         std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld: %s after %d rounds, %.3f s\n",
                      static_cast<long long>(g.mg), static_cast<long long>(g.ng), static_cast<long long>(kk),
@@ -361,9 +362,10 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
   // uncomputed tail enters the discarded weight, which is exactly the sum
   // the full rule would accumulate over those values as long as every one
   // of them lies below the kept set.  That holds when each truncated
-  // group's smallest computed value is below the last kept value by more
-  // than the 1e-3 degeneracy gap; otherwise the group is decomposed in full
-  // and the rule is applied again.
+  // group's smallest computed value, raised by that triplet's residual (the
+  // value is only known to lie within it of a singular value), is below the
+  // last kept value by more than the 1e-3 degeneracy gap; otherwise the
+  // group is decomposed in full and the rule is applied again.
   struct SV { double v; int g; int64_t i; };
   std::vector<SV> all;
   size_t length = 0;
@@ -396,7 +398,7 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
     bool redo = false;
     const double last = all[length - 1].v;
     for (Group* g : glist)
-      if (g->truncated && !(g->S.back() < last * (1.0 - 2.0e-3))) {
+      if (g->truncated && !(g->S.back() + g->last_res < last * (1.0 - 2.0e-3))) {
         full_svd(*g);   // each pass turns at least one group exact: at most #groups passes
         redo = true;
       }

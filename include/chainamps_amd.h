@@ -137,6 +137,26 @@ int cam_heff_apply_blocked32(cam_heff* h, const float* in, float* out);
  * (L, W, W, R), the average launch duration in ms measured with HIP events
  * on the context stream and the algorithmic flops of that stage. */
 int cam_heff_profile(cam_heff* h, int precision, int reps, double* stage_ms, double* stage_flops);
+/* The same on the caller's device block-layout vectors (FP64, or FP32
+ * split complex for CAM_FP32), for sizes where internal vectors do not fit
+ * next to the caller's. */
+int cam_heff_profile_device(cam_heff* h, int precision, int reps, const void* in, void* out, double* stage_ms,
+                            double* stage_flops);
+/* Sliced matvec (no reference counterpart: TwoVertexMPOMPSMultiply,
+ * DMRG_routines.cpp:898-974, forms LeftPart.V whole).  The a0' rows of
+ * every sector are cut into row_slices ranges and the a2 columns into
+ * col_slices; each slice runs the four stages on intermediates of
+ * 1/(row_slices col_slices) the size, and the column slices accumulate
+ * into the output.  Results are the same up to the order of the a2 sum.
+ * By default cam_heff_create takes the fewest slices whose two
+ * intermediate buffers fit the free device memory (plus the context's
+ * existing workspaces, less 4 GB): 1 x 1 wherever the whole intermediates
+ * fit.  set_slices fixes the counts; set_workspace_limit takes the fewest
+ * (powers of two) whose buffers fit `bytes`; get_slices reports the counts
+ * and the workspace bytes.  The adjoint needs 1 x 1. */
+int cam_heff_set_slices(cam_heff* h, int row_slices, int col_slices);
+int cam_heff_set_workspace_limit(cam_heff* h, double bytes);
+int cam_heff_get_slices(const cam_heff* h, int* row_slices, int* col_slices, double* workspace_bytes);
 /* Algorithmic HBM bytes per stage (L, W, W, R) of the FP64 matvec: every
  * operand read once and the output written once (psi -> T1 -> T2 -> T3 ->
  * out, plus the environment or MPO blocks), 16 bytes per element. */
@@ -222,15 +242,45 @@ int cam_bond_evolution_operator(cam_ctx* ctx, const cam_tensor* H, double timest
  * MPO, target the target charges (nrules shorts).  converge=0 is the
  * reference growth-step eigensolver setting (tol 0.1); mixed=1 runs the
  * bulk of each eigensolve in FP32.  Each cam_idmrg_grow call adds two
- * vertices; record[16] receives {N, energy, energy/N, entropy,
- * truncation, chi kept, n, matvecs fp64, matvecs fp32, residual, seconds,
- * heff flops, eigensolve seconds, SVD seconds, block update + H_eff build
- * + prediction seconds, 1 - fidelity of the prediction (CheckConvergence,
- * DMRG_routines.cpp:769-771, the Energies.dat column)}. */
+ * vertices; record[CAM_IDMRG_RECORD] receives {N, energy, energy/N,
+ * entropy, truncation, chi kept, n, matvecs fp64, matvecs fp32, residual,
+ * seconds, heff flops, eigensolve seconds, SVD seconds, block update +
+ * H_eff build + prediction seconds, 1 - fidelity of the prediction
+ * (CheckConvergence, DMRG_routines.cpp:769-771, the Energies.dat column),
+ * 1 when no prediction was made for the next step (the new bond's two
+ * sides carry different charges, i.e. a non-zero target; that step then
+ * starts at random), vertices in the left block, vertices in the right
+ * block}. */
+#define CAM_IDMRG_RECORD 19
 typedef struct cam_idmrg_s cam_idmrg;
 int cam_idmrg_create(cam_ctx* ctx, const cam_tensor* W, const int16_t* target, int converge,
                      int mixed, int ncv, cam_idmrg** out);
 int cam_idmrg_grow(cam_idmrg* d, int64_t chi, double truncation, double* record);
+/* ---- finite DMRG sweeps ---------------------------------------------------
+ * One finite step on the grown chain: SuperBlock::move_right_two_vertex
+ * (to_right=1) or move_left_two_vertex (DMRG_routines.cpp:376-444).  The
+ * vertex leaving the centre joins the block on that side (MakeLeftBlock /
+ * MakeRightBlock, or the initial block at an end), the prediction is
+ * MakeRFinitePrediction / MakeLFinitePrediction (709-716) from the stored
+ * MPS matrix of the vertex that enters, the eigensolve uses the object's
+ * converge / mixed / ncv setting, the SVD keeps chi states with the
+ * truncation switched off next to an end (-0.0), and the record's fidelity
+ * field is 1 - |<prediction|U s Vd>| (CheckConvergence(Prediction,
+ * MPSDecomposition), 773-776).  Blocks and MPS matrices of every position
+ * stay on the device (the reference's .BLOCK / .MPS_matrix files).  Growth
+ * after a finite step is an error. */
+int cam_idmrg_move(cam_idmrg* d, int to_right, int64_t chi, double truncation, double* record);
+/* FiniteDMRG::run (fDMRG_routines.cpp:19-66): num_sweeps sweeps, each
+ * moving right to the end, left to the other end and right back to the
+ * middle; records receives CAM_IDMRG_RECORD doubles per step and midpoint
+ * (optional) 1 for the steps the reference writes to Energies.dat (centre
+ * in the middle, on the way left and on the way back).  With records NULL
+ * only *nrec (the number of steps) is set. */
+int cam_fdmrg_sweeps(cam_idmrg* d, int64_t num_sweeps, int64_t chi, double truncation, double* records,
+                     int32_t* midpoint, int64_t cap, int64_t* nrec);
+/* the left and right blocks of the current centre (new handles sharing the
+ * device tensors; destroy them with cam_tensor_destroy) */
+int cam_idmrg_blocks(const cam_idmrg* d, cam_tensor** LB, cam_tensor** RB);
 /* singular values of the current central decomposition (capacity cap) */
 int cam_idmrg_values(const cam_idmrg* d, double* values, int64_t cap, int64_t* n);
 int cam_idmrg_destroy(cam_idmrg* d);

@@ -503,10 +503,29 @@ class TEBD {
 
 class UnitCell;
 
-/* iDMRG growth (iDMRG::run / SuperBlock::grow_two_vertex). */
+/* iDMRG growth (iDMRG::run / SuperBlock::grow_two_vertex) and finite
+ * steps (SuperBlock::move_right/left_two_vertex): one Energies.dat row
+ * (Energy, Energy/vertices, Entropy, Truncation, Fidelity) and the centre's
+ * position. */
 struct iDMRGStep {
-  MPXInt N = 0, chi = 0, n = 0;
-  double Energy = 0.0, EnergyPerVertex = 0.0, Entropy = 0.0, Truncation = 0.0, Seconds = 0.0;
+  MPXInt N = 0, chi = 0, n = 0, left = 0, right = 0;
+  double Energy = 0.0, EnergyPerVertex = 0.0, Entropy = 0.0, Truncation = 0.0, Fidelity = 0.0,
+         Seconds = 0.0;
+  static iDMRGStep from_record(const double* r) {
+    iDMRGStep s;
+    s.N = static_cast<MPXInt>(r[0]);
+    s.Energy = r[1];
+    s.EnergyPerVertex = r[2];
+    s.Entropy = r[3];
+    s.Truncation = r[4];
+    s.chi = static_cast<MPXInt>(r[5]);
+    s.n = static_cast<MPXInt>(r[6]);
+    s.Seconds = r[10];
+    s.Fidelity = r[15];
+    s.left = static_cast<MPXInt>(r[17]);
+    s.right = static_cast<MPXInt>(r[18]);
+    return s;
+  }
 };
 class iDMRG {
  public:
@@ -522,22 +541,46 @@ class iDMRG {
    * growth step (iDMRG_DRV.cpp:70). */
   inline UnitCell CentralUnitCell() const;
   iDMRGStep grow(MPXInt chi, double truncation = 0.0) {
-    double r[16];
+    double r[CAM_IDMRG_RECORD];
     check(cam_idmrg_grow(d_.get(), chi, truncation, r));
-    iDMRGStep s;
-    s.N = static_cast<MPXInt>(r[0]);
-    s.Energy = r[1];
-    s.EnergyPerVertex = r[2];
-    s.Entropy = r[3];
-    s.Truncation = r[4];
-    s.chi = static_cast<MPXInt>(r[5]);
-    s.n = static_cast<MPXInt>(r[6]);
-    s.Seconds = r[10];
-    return s;
+    return iDMRGStep::from_record(r);
   }
+  /* SuperBlock::move_right_two_vertex / move_left_two_vertex
+   * (DMRG_routines.cpp:376-444) on the grown chain */
+  iDMRGStep move_right_two_vertex(MPXInt chi, double truncation = 0.0) { return move(true, chi, truncation); }
+  iDMRGStep move_left_two_vertex(MPXInt chi, double truncation = 0.0) { return move(false, chi, truncation); }
+
+ protected:
+  std::shared_ptr<cam_idmrg> d_;
 
  private:
-  std::shared_ptr<cam_idmrg> d_;
+  iDMRGStep move(bool right, MPXInt chi, double truncation) {
+    double r[CAM_IDMRG_RECORD];
+    check(cam_idmrg_move(d_.get(), right, chi, truncation, r));
+    return iDMRGStep::from_record(r);
+  }
+};
+
+/* FiniteDMRG (fDMRG_routines.hpp): growth to the chain length with
+ * iDMRG, then FiniteDMRG::run (fDMRG_routines.cpp:19-66), which sweeps the
+ * centre right to the end, left to the other end and back to the middle;
+ * run() returns the rows the reference writes to Energies.dat (the centre
+ * in the middle, on the way left and on the way back). */
+class FiniteDMRG : public iDMRG {
+ public:
+  FiniteDMRG(const MPX_matrix& H, const State& target, bool converge = false, bool mixed = false)
+      : iDMRG(H, target, converge, mixed) {}
+  std::vector<iDMRGStep> run(MPXInt num_sweeps, MPXInt chi, double truncation = 0.0) {
+    int64_t n = 0;
+    check(cam_fdmrg_sweeps(d_.get(), num_sweeps, chi, truncation, nullptr, nullptr, 0, &n));
+    std::vector<double> r(static_cast<size_t>(n > 0 ? n : 1) * CAM_IDMRG_RECORD);
+    std::vector<int32_t> mid(static_cast<size_t>(n > 0 ? n : 1));
+    check(cam_fdmrg_sweeps(d_.get(), num_sweeps, chi, truncation, r.data(), mid.data(), n, &n));
+    std::vector<iDMRGStep> out;
+    for (int64_t i = 0; i < n; ++i)
+      if (mid[i]) out.push_back(iDMRGStep::from_record(r.data() + i * CAM_IDMRG_RECORD));
+    return out;
+  }
 };
 
 /* UnitCell (UnitCell.hpp:29-66) on the device: Matrices [s, a_l, a_r] and

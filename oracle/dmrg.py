@@ -302,6 +302,35 @@ def check_convergence(aux, prev_lambda):
     return 1.0 - float(np.sum(np.linalg.svd(m, compute_uv=False)))
 
 
+def make_l_finite_prediction(U: MPX, values, A: MPX):
+    """MakeLFinitePrediction (DMRG_routines.cpp:709-712): the guess of a
+    step to the left, A . (U . diag(values)) with A the stored left matrix
+    of the vertex that joins the centre; a sparse vector of the full
+    (d chiL d chiR) index in the mixed radix."""
+    lam = diagonal_mpx(U.rules, U.indices[2], U.indices[2], values)
+    return reshape_to_vector(contract(A, False, contract(U, False, lam, False, [(2, 0)]), False, [(2, 1)]))
+
+
+def make_r_finite_prediction(U: MPX, values, Vd: MPX, B: MPX):
+    """MakeRFinitePrediction (DMRG_routines.cpp:713-716): the guess of a
+    step to the right, reorder(diag(values) . Vd, 102, 2) . B with B the
+    stored right matrix of the vertex that joins the centre."""
+    lam = diagonal_mpx(U.rules, U.indices[2], U.indices[2], values)
+    t = reorder(contract(lam, False, Vd, False, [(1, 0)]), False, [1, 0, 2], 2)
+    return reshape_to_vector(contract(t, False, B, False, [(2, 0)]))
+
+
+def check_convergence_finite(guess, U: MPX, values, Vd: MPX):
+    """CheckConvergence(Prediction, MPSDecomposition) (DMRG_routines.cpp:
+    773-776): 1 - |trace(reshape(guess)^dagger (U diag(values) Vd))|, i.e.
+    1 - |<guess|psi>| with psi the truncated, rescaled decomposition."""
+    lam = diagonal_mpx(U.rules, Vd.indices[0], Vd.indices[0], values)
+    psi = contract_to_sparse(contract(U, False, lam, False, [(2, 0)]), False, Vd, False, [(2, 0)])
+    g = np.asarray(guess.toarray()).reshape(-1)
+    x = np.asarray(psi.toarray()).reshape(-1, order="F")
+    return 1.0 - abs(np.vdot(g, x))
+
+
 def entropy(values):
     """sparse_interface.cpp:76-85."""
     v = np.asarray(values, dtype=np.float64)
@@ -321,6 +350,12 @@ class IDMRG:
         self.converge = converge
         self.size = 0
         self.history = []
+        # block store of the finite algorithm (the reference's .BLOCK and
+        # .MPS_matrix files, BlocksStructure DMRG_routines.cpp:57-186):
+        # lblk[k] / rblk[k] hold k vertices, lmps[k] / rmps[k] are the MPS
+        # matrices of vertex k counted from the left / right end
+        self.lblk, self.rblk, self.lmps, self.rmps = {}, {}, {}, {}
+        self.nleft = self.nright = 0
 
     def _svd(self, psi, chi, trunc):
         r = mpx_svd(psi, chi, trunc)
@@ -342,6 +377,8 @@ class IDMRG:
         self.LB = make_initial_left_block(self.LeftH, U)
         self.RB = make_initial_right_block(self.RightH, Vd)
         self.size = 2
+        self.lblk[1], self.rblk[1] = self.LB, self.RB
+        self.lmps[1], self.rmps[1] = U, Vd
         self.prev_lambda = np.array([1.0])
         self.guess, self.aux = make_prediction(U, vals, Vd, self.prev_lambda)
         fid = check_convergence(self.aux, self.prev_lambda)
@@ -356,7 +393,9 @@ class IDMRG:
         if self.size > 2:
             self.LB = make_left_block(self.LB, self.W, self.U)
             self.RB = make_right_block(self.RB, self.W, self.Vd)
+            self.lblk[self.size // 2], self.rblk[self.size // 2] = self.LB, self.RB
         self.size += 2
+        self.nleft = self.nright = self.size // 2 - 1
         self.prev_lambda = self.values
         comp = TwoVertexComponents(self.LB, self.W, self.RB)
         g = np.asarray(self.guess.toarray()).reshape(-1)[comp.allowed]
@@ -364,6 +403,7 @@ class IDMRG:
         psi = psi_from_vector(comp, vec)
         U, vals, Vd, tr = self._svd(psi, chi, trunc)
         self.U, self.values, self.Vd, self.trunc = U, vals, Vd, tr
+        self.lmps[self.size // 2], self.rmps[self.size // 2] = U, Vd
         self.guess, self.aux = make_prediction(U, vals, Vd, self.prev_lambda)
         fid = check_convergence(self.aux, self.prev_lambda)
         rec = dict(N=self.size, energy=E, energy_per_vertex=E / self.size, entropy=entropy(vals),
@@ -375,3 +415,82 @@ class IDMRG:
         for _ in range(steps):
             self.grow(chi, trunc)
         return self.history
+
+    # -- finite sweeps (DMRG_routines.cpp:376-444, fDMRG_routines.cpp:19-66)
+    def _finite_solve(self, guess, chi, trunc):
+        comp = TwoVertexComponents(self.lblk_at(self.nleft), self.W, self.rblk_at(self.nright))
+        g = np.asarray(guess.toarray()).reshape(-1)[comp.allowed]
+        E, vec = comp.hed(guess=g, converge=self.converge)
+        U, vals, Vd, tr = self._svd(psi_from_vector(comp, vec), chi, trunc)
+        return E, U, vals, Vd, tr, comp.n
+
+    def lblk_at(self, k):
+        return make_dummy_left_block(self.W, self.target) if k == 0 else self.lblk[k]
+
+    def rblk_at(self, k):
+        return make_dummy_right_block(self.W, self.target) if k == 0 else self.rblk[k]
+
+    def move_right(self, chi, trunc=0.0):
+        """SuperBlock::move_right_two_vertex (DMRG_routines.cpp:376-409)."""
+        if self.nright < 1:
+            raise ValueError("At right end of superblock already...")
+        if self.nleft < 1:
+            newleft = make_initial_left_block(self.LeftH, self.U)
+        else:
+            newleft = make_left_block(self.lblk[self.nleft], self.W, self.U)
+        self.nleft += 1
+        self.nright -= 1
+        self.lblk[self.nleft] = newleft
+        self.prev_lambda = self.values
+        guess = make_r_finite_prediction(self.U, self.values, self.Vd, self.rmps[self.nright + 1])
+        t = -0.0 if self.nright <= 1 else trunc
+        return self._finish_move(guess, chi, t)
+
+    def move_left(self, chi, trunc=0.0):
+        """SuperBlock::move_left_two_vertex (DMRG_routines.cpp:411-444)."""
+        if self.nleft < 1:
+            raise ValueError("At left end of superblock already...")
+        if self.nright < 1:
+            newright = make_initial_right_block(self.RightH, self.Vd)
+        else:
+            newright = make_right_block(self.rblk[self.nright], self.W, self.Vd)
+        self.nright += 1
+        self.nleft -= 1
+        self.rblk[self.nright] = newright
+        self.prev_lambda = self.values
+        guess = make_l_finite_prediction(self.U, self.values, self.lmps[self.nleft + 1])
+        t = -0.0 if self.nleft <= 1 else trunc
+        return self._finish_move(guess, chi, t)
+
+    def _finish_move(self, guess, chi, trunc):
+        E, U, vals, Vd, tr, n = self._finite_solve(guess, chi, trunc)
+        self.U, self.values, self.Vd, self.trunc = U, vals, Vd, tr
+        self.lmps[self.nleft + 1], self.rmps[self.nright + 1] = U, Vd
+        fid = check_convergence_finite(guess, U, vals, Vd)
+        return dict(N=self.size, left=self.nleft, right=self.nright, energy=E,
+                    energy_per_vertex=E / self.size, entropy=entropy(vals), truncation=tr,
+                    fidelity=fid, chi=len(vals), n=n)
+
+    def finite_sweeps(self, num_sweeps, chi, trunc=0.0):
+        """FiniteDMRG::run (fDMRG_routines.cpp:19-66): each sweep moves right
+        to the end, left to the other end and right back to the middle;
+        returns (every step's record, the records taken at the midpoint,
+        i.e. the rows the reference pushes to Energies.dat)."""
+        steps, mid = [], []
+        if self.size == 2:
+            return steps, mid
+        for _ in range(num_sweeps):
+            for _ in range(self.nright):
+                steps.append(self.move_right(chi, trunc))
+            for _ in range(self.nleft):
+                r = self.move_left(chi, trunc)
+                steps.append(r)
+                if self.nleft == self.nright:
+                    mid.append(r)
+            # for (r = right_size(); r > left_size(); --r): both sides move
+            for _ in range((self.nright - self.nleft + 1) // 2):
+                r = self.move_right(chi, trunc)
+                steps.append(r)
+                if self.nleft == self.nright:
+                    mid.append(r)
+        return steps, mid

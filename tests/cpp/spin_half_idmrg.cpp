@@ -66,7 +66,7 @@ int main() {
   const double energy[5] = {-1.0, -3.0, -5.0, -7.0, -9.0};
   const double entropy[5] = {0.6931471805599454, 0.8675632284814612, 1.018230153951394, 1.138073514962318,
                              1.235866276237509};
-  iDMRG dmrg(H, State{0}, /*converge=*/false);
+  FiniteDMRG dmrg(H, State{0}, /*converge=*/false);
   for (int step = 0; step < 5; ++step) {
     const iDMRGStep r = dmrg.grow(50);
     char what[64];
@@ -74,6 +74,20 @@ int main() {
     expect(std::fabs(r.Energy - energy[step]) < 1e-10, what, r.Energy, energy[step]);
     std::snprintf(what, sizeof what, "iDMRG N=%lld entropy", r.N);
     expect(std::fabs(r.Entropy - entropy[step]) < 1e-9, what, r.Entropy, entropy[step]);
+  }
+  // fDMRG_DRV -B 50 -F 2 (README.md:29-40): two finite sweeps of the ten
+  // vertices; the midpoint rows (iterations 5-8) keep E = -9 and the
+  // midpoint entropy 1.2358662762375
+  {
+    const std::vector<iDMRGStep> rows = dmrg.run(2, 50);
+    expect(rows.size() == 4, "fDMRG midpoint rows", static_cast<double>(rows.size()), 4.0);
+    for (const iDMRGStep& r : rows) {
+      char what[80];
+      std::snprintf(what, sizeof what, "fDMRG midpoint (%lld,%lld) energy", r.left, r.right);
+      expect(r.left == 4 && r.right == 4 && std::fabs(r.Energy + 9.0) < 1e-10, what, r.Energy, -9.0);
+      std::snprintf(what, sizeof what, "fDMRG midpoint (%lld,%lld) entropy", r.left, r.right);
+      expect(std::fabs(r.Entropy - 1.235866276237508) < 1e-9, what, r.Entropy, 1.235866276237508);
+    }
   }
 
   // the iDMRG driver's measurement step (iDMRG_DRV.cpp:64-82): Orthogonalise

@@ -3,9 +3,11 @@
 Reads (as data, nothing is executed) examples/user_defined/spin_half_example/
 {vertex_basis.in, Sx.in, Sy.in, Sz.in, example_input, README.md} and writes
 spin_half_example.json: the basis charges, the operator triplets, the
-couplings of example_input and the growth-step rows (Index, Energy,
+couplings of example_input, the growth-step rows (Index, Energy,
 Energy/vertices, Entropy) of example_input_Energies.dat quoted in
-README.md:31-40 (growth steps 0-4 = 2..10 sites).
+README.md:31-40 (growth steps 0-4 = 2..10 sites) and its finite-sweep rows
+(iterations 5-8: the midpoint rows of the two sweeps of `-F 2`, README.md:29,
+with the Truncation and Fidelity columns).
 """
 import json
 import re
@@ -47,11 +49,12 @@ def main():
         if l.startswith("~~~~"):
             break
         p = l.split()
-        rows.append([int(p[0]), float(p[1]), float(p[2]), float(p[3])])
-    growth = rows[:5]   # iterations 0-4 are the growth steps (README.md:42)
-    out = {"source": "examples/user_defined/spin_half_example (README.md:31-40, *.in, example_input)",
+        rows.append([int(p[0])] + [float(v) for v in p[1:6]])
+    growth = [r[:4] for r in rows[:5]]   # iterations 0-4 are the growth steps (README.md:42)
+    finite = rows[5:]                    # iterations 5-8: finite-size sweeps
+    out = {"source": "examples/user_defined/spin_half_example (README.md:29-42, *.in, example_input)",
            "basis": basis, "operators": ops, "couplings": couplings,
-           "chi": 50, "growth_steps": growth}
+           "chi": 50, "growth_steps": growth, "sweeps": 2, "finite_sweep_rows": finite}
     Path(__file__).with_name("spin_half_example.json").write_text(json.dumps(out, indent=1))
 
 

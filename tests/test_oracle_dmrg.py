@@ -111,8 +111,8 @@ def test_idmrg_heisenberg_matches_exact_diagonalisation():
         assert abs(rec["energy"] - heis_ed(N)) < 1e-10
 
 
-def test_spin_half_example_fixture():
-    g = json.load(open(os.path.join(GOLD, "spin_half_example.json")))
+def spin_half_example_mpo(g):
+    """MakeGeneralHMPO of the fixture's model (example_input couplings)."""
     rules = ChargeRules([0])
     phys = np.array([[b[1]] for b in g["basis"]])
     d = len(phys)
@@ -128,7 +128,12 @@ def test_spin_half_example_fixture():
             local.append((c["value"], ops[c["ops"][0]]))
         else:
             coup.append((c["value"], ops[c["ops"][0]], ops[c["ops"][1]]))
-    W = models.general_hmpo(rules, phys, np.zeros((d, d)), local, coup)
+    return models.general_hmpo(rules, phys, np.zeros((d, d)), local, coup)
+
+
+def test_spin_half_example_fixture():
+    g = json.load(open(os.path.join(GOLD, "spin_half_example.json")))
+    W = spin_half_example_mpo(g)
     assert W.dims() == [2, 11, 2, 11] and W.is_consistent()
     idm = dmrg.IDMRG(W, converge=False)
     for (index, energy, per_vertex, entropy) in g["growth_steps"]:
@@ -136,6 +141,55 @@ def test_spin_half_example_fixture():
         assert abs(rec["energy"] - energy) < 1e-12
         assert abs(rec["energy_per_vertex"] - per_vertex) < 1e-12
         assert abs(rec["entropy"] - entropy) < 1e-12
+
+
+def test_spin_half_example_finite_sweeps():
+    """fDMRG_DRV -B 50 -F 2 on the ten vertices (README.md:29-40): the
+    midpoint rows of the two sweeps (iterations 5-8) reproduce the
+    README's energies and entanglement entropies (1e-12).  Every finite
+    step's prediction is exact here (no truncation), so 1 - |<guess|psi>|
+    is 0 to rounding; the README's Fidelity column reads exactly 1 for these
+    rows, a value the CheckConvergence formula (DMRG_routines.cpp:773-776)
+    cannot take for an exact prediction -- parity unpinned (see the
+    oracle's check_convergence_finite)."""
+    g = json.load(open(os.path.join(GOLD, "spin_half_example.json")))
+    W = spin_half_example_mpo(g)
+    idm = dmrg.IDMRG(W, converge=False)
+    for _ in g["growth_steps"]:
+        idm.grow(g["chi"])
+    steps, mid = idm.finite_sweeps(g["sweeps"], g["chi"])
+    assert len(steps) == 2 * 2 * (10 - 2) and len(mid) == len(g["finite_sweep_rows"])
+    for r, (index, energy, per_vertex, entropy, trunc, fid) in zip(mid, g["finite_sweep_rows"]):
+        assert (r["left"], r["right"]) == (4, 4)
+        assert abs(r["energy"] - energy) < 1e-12
+        assert abs(r["energy_per_vertex"] - per_vertex) < 1e-12
+        assert abs(r["entropy"] - entropy) < 1e-12
+        assert r["truncation"] == trunc == 0.0
+    assert all(abs(r["fidelity"]) < 1e-12 for r in steps)
+    # the centre visits both ends: (8,0) and (0,8)
+    assert {(r["left"], r["right"]) for r in steps} >= {(8, 0), (0, 8)}
+
+
+def test_finite_sweeps_match_exact_diagonalisation():
+    """Without truncation every finite step of an N = 8 Heisenberg chain
+    returns the exact open-chain ground energy (1e-10); with a small bond
+    dimension the energies stay above it (variational) and the truncated
+    steps report a non-zero discarded weight."""
+    rules, phys, W = models.heisenberg_mpo()
+    e8 = heis_ed(8)
+    idm = dmrg.IDMRG(W, converge=True)
+    for _ in range(4):
+        idm.grow(1024)
+    steps, mid = idm.finite_sweeps(1, 1024)
+    assert len(mid) == 2
+    for r in steps:
+        assert abs(r["energy"] - e8) < 1e-10, r
+    small = dmrg.IDMRG(W, converge=True)
+    for _ in range(4):
+        small.grow(4)
+    steps, _ = small.finite_sweeps(1, 4, 1e-14)
+    assert all(r["energy"] >= e8 - 1e-10 for r in steps)
+    assert any(r["truncation"] > 0 for r in steps)
 
 
 def test_heff_hermitian_for_physical_model():
