@@ -73,6 +73,7 @@ _SIGS = {
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
     "cam_heff_apply_blocked32": [_p, _p, _p],
     "cam_heff_set_projectors": [_p, C.c_int, C.POINTER(_dp), _dp],
+    "cam_ctx_release_workspaces": [_p],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
     "cam_heff_profile_device": [_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp],
     "cam_heff_set_slices": [_p, C.c_int, C.c_int],
@@ -133,6 +134,10 @@ class Context:
         h = _p()
         _check(lib.cam_ctx_create(device, C.byref(h)))
         self.h = h
+
+    def release_workspaces(self):
+        """Free the grow-only Lanczos and H_eff workspaces of the context."""
+        _check(_LIB.cam_ctx_release_workspaces(self.h))
 
     def sync(self):
         _check(_LIB.cam_ctx_sync(self.h))

@@ -78,6 +78,17 @@ int cam_ctx_sync(cam_ctx* c) {
   CAM_API_END
 }
 
+int cam_ctx_release_workspaces(cam_ctx* c) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(c, "null argument");
+  set_device(c);
+  HIP_CHECK(hipStreamSynchronize(c->ctx.stream));
+  c->ctx.ws_krylov.reset();
+  c->ctx.ws_t1.reset();
+  c->ctx.ws_t2.reset();
+  CAM_API_END
+}
+
 int cam_ctx_stream(cam_ctx* c, void** stream) {
   CAM_API_BEGIN
   *stream = reinterpret_cast<void*>(c->ctx.stream);

@@ -274,13 +274,20 @@ void Heff::plan_slices(int rs, int cs) {
       // keeps rows [r0, r0+mr) of its a0' sector and columns [c0, c0+mc) of
       // its a2 sector, dense over (a0', a2, sx, sy)
       struct SB { int64_t off, r0, mr, c0, mc, n1, n2; };
+      // (blocks in the order of the whole layout, so that 1 x 1 reproduces
+      // it exactly: the adjoint reads T1 and T2 at those offsets)
       auto layout = [&](const std::map<std::array<int, 5>, TB>& T, std::map<std::array<int, 5>, SB>& out) {
+        std::vector<std::pair<int64_t, const std::array<int, 5>*>> order;
+        order.reserve(T.size());
+        for (auto& kv : T) order.emplace_back(kv.second.off, &kv.first);
+        std::sort(order.begin(), order.end());
         int64_t tot = 0;
-        for (auto& kv : T) {
-          SB b{tot, 0, 0, 0, 0, kv.second.n1, kv.second.n2};
-          slice_range(kv.second.m0, ri, rs, b.r0, b.mr);
-          slice_range(kv.second.m3, ci, cs, b.c0, b.mc);
-          out[kv.first] = b;
+        for (auto& o : order) {
+          const TB& tb = T.at(*o.second);
+          SB b{tot, 0, 0, 0, 0, tb.n1, tb.n2};
+          slice_range(tb.m0, ri, rs, b.r0, b.mr);
+          slice_range(tb.m3, ci, cs, b.c0, b.mc);
+          out[*o.second] = b;
           tot += b.mr * b.mc * b.n1 * b.n2;
         }
         return tot;
@@ -506,6 +513,7 @@ void Heff::add_projectors32(const float2* in, float2* out) {
 
 void Heff::forward_stages(const cplx* in, int nstages) {
   CAM_REQUIRE(nslices() == 1, "heff: the adjoint needs the whole intermediates (an unsliced matvec)");
+  CAM_REQUIRE(ws1n == std::max(t1n, t3n) && ws2n == t2n, "heff: unsliced plan layout differs from the intermediates");
   const GemmBases b = bases(0, in, nullptr);
   for (int i = 0; i < std::min(nstages, 3); ++i) launch_grouped_gemm(plans[0][i], 0, ctx->stream, &b);
 }

@@ -93,8 +93,9 @@ def _hermitian_channels(T, mask, rows, cols):
     return np.where(mask, np.moveaxis(M, [0, 1, 2], [1, 0, 2]).reshape(cols), 0)
 
 
-def random_left_block(ctx, rules, aq, bq, rng, hermitian=False):
-    """[dummy(out), a'(in), dummy(in), b(out), dummy(in), a(out)]."""
+def left_block_dense(rules, aq, bq, rng, hermitian=False):
+    """(index descriptors, dense tensor) of a random left block
+    [dummy(out), a'(in), dummy(in), b(out), dummy(in), a(out)]."""
     z = np.zeros((1, len(rules)), np.int64)
     idx = [IndexDesc(False, False, z), IndexDesc(True, False, aq), IndexDesc(True, False, z),
            IndexDesc(False, False, bq), IndexDesc(True, False, z), IndexDesc(False, False, aq)]
@@ -103,11 +104,12 @@ def random_left_block(ctx, rules, aq, bq, rng, hermitian=False):
     T = np.where(mask, rng.standard_normal(dims) + 1j * rng.standard_normal(dims), 0)
     if hermitian:
         T = _hermitian_channels(T, mask, (len(aq), len(bq), len(aq)), dims)
-    return tensor_from_dense(ctx, rules, idx, 3, T)
+    return idx, T
 
 
-def random_right_block(ctx, rules, aq, bq, rng, hermitian=False):
-    """[a'(out), dummy(in), b(in), dummy(out), a(in), dummy(out)]."""
+def right_block_dense(rules, aq, bq, rng, hermitian=False):
+    """(index descriptors, dense tensor) of a random right block
+    [a'(out), dummy(in), b(in), dummy(out), a(in), dummy(out)]."""
     z = np.zeros((1, len(rules)), np.int64)
     idx = [IndexDesc(False, False, aq), IndexDesc(True, False, z), IndexDesc(True, False, bq),
            IndexDesc(False, False, z), IndexDesc(True, False, aq), IndexDesc(False, False, z)]
@@ -116,6 +118,16 @@ def random_right_block(ctx, rules, aq, bq, rng, hermitian=False):
     T = np.where(mask, rng.standard_normal(dims) + 1j * rng.standard_normal(dims), 0)
     if hermitian:
         T = _hermitian_channels(T, mask, (len(aq), len(bq), len(aq)), dims)
+    return idx, T
+
+
+def random_left_block(ctx, rules, aq, bq, rng, hermitian=False):
+    idx, T = left_block_dense(rules, aq, bq, rng, hermitian)
+    return tensor_from_dense(ctx, rules, idx, 3, T)
+
+
+def random_right_block(ctx, rules, aq, bq, rng, hermitian=False):
+    idx, T = right_block_dense(rules, aq, bq, rng, hermitian)
     return tensor_from_dense(ctx, rules, idx, 2, T)
 
 
@@ -142,10 +154,12 @@ def sector_profile(chi=1024, qmin=-10, qmax=10, sigma=4.0):
     return qs, base
 
 
-def u1_batching_inputs(ctx, chi=1024, d=32, w=6, seed=0):
-    """21 U(1) sectors (charges -10..10, Gaussian sizes, sigma 4), d charges
-    uniform in [-3,3], MPO bond charges (0,+1,-1,+2,-2,0)[:w], all blocks
-    complex N(0,1) on QN-allowed entries."""
+def u1_batching_dense(chi=1024, d=32, w=6, seed=0):
+    """The batching workload on the host: 21 U(1) sectors (charges -10..10,
+    Gaussian sizes, sigma 4), d charges uniform in [-3,3], MPO bond charges
+    (0,+1,-1,+2,-2,0)[:w], all blocks complex N(0,1) on QN-allowed entries.
+    Returns rules and (index descriptors, dense tensor, row indices) of L,
+    W and R."""
     rng = np.random.default_rng(seed)
     rules = [0]
     qs, sizes = sector_profile(chi)
@@ -157,7 +171,12 @@ def u1_batching_inputs(ctx, chi=1024, d=32, w=6, seed=0):
     dims = [d, w, d, w]
     mask = _flux_mask(rules, idx, dims)
     T = np.where(mask, rng.standard_normal(dims) + 1j * rng.standard_normal(dims), 0)
-    W = tensor_from_dense(ctx, rules, idx, 2, T)
-    L = random_left_block(ctx, rules, aq, bq, rng)
-    R = random_right_block(ctx, rules, aq, bq, rng)
-    return L, W, R
+    li, lt = left_block_dense(rules, aq, bq, rng)
+    ri, rt = right_block_dense(rules, aq, bq, rng)
+    return rules, (li, lt, 3), (idx, T, 2), (ri, rt, 2)
+
+
+def u1_batching_inputs(ctx, chi=1024, d=32, w=6, seed=0):
+    """The batching workload (u1_batching_dense) as device tensors L, W, R."""
+    rules, *parts = u1_batching_dense(chi, d, w, seed)
+    return tuple(tensor_from_dense(ctx, rules, idx, nr, T) for idx, T, nr in parts)

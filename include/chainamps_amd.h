@@ -48,6 +48,10 @@ enum { CAM_FP64 = 0, CAM_FP32 = 1 };
 int cam_ctx_create(int device, cam_ctx** out);
 int cam_ctx_destroy(cam_ctx* ctx);
 int cam_ctx_sync(cam_ctx* ctx);
+/* Frees the context's grow-only workspaces (the Lanczos basis and the two
+ * H_eff intermediate buffers; the next call that needs them allocates them
+ * again), e.g. before a call that needs the memory for something else. */
+int cam_ctx_release_workspaces(cam_ctx* ctx);
 /* The hipStream_t the context launches on (for events/profiling). */
 int cam_ctx_stream(cam_ctx* ctx, void** stream);
 const char* cam_last_error(void);

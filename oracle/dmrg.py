@@ -141,6 +141,36 @@ class TwoVertexComponents:
         return val, gauge_fix(vec)
 
 
+class StagewiseHeff:
+    """H_eff . v on dense tensors in the four-stage order L . psi, W on
+    sigma1, W on sigma2, . R (BLAS tensordots), restricted to the allowed
+    entries of ``comp``: an independent restatement of
+    TwoVertexMPOMPSMultiply (DMRG_routines.cpp:898-974), pinned against the
+    reference-order matvec in tests/test_oracle_dmrg.py and fast enough for
+    converged eigensolves at d = 64."""
+
+    def __init__(self, LB: MPX, W: MPX, RB: MPX, comp: TwoVertexComponents):
+        self.L = LB.to_dense_tensor()[0, :, 0, :, 0, :]        # (a0', b, a0)
+        self.R = RB.to_dense_tensor()[:, 0, :, 0, :, 0]        # (a2', b, a2)
+        self.W = W.to_dense_tensor()                           # (s', bl, s, br)
+        self.d = self.W.shape[0]
+        self.chiL, self.chiR = self.L.shape[0], self.R.shape[0]
+        self.allowed = comp.allowed
+        self.length = comp.length
+        self.n = comp.n
+
+    def matvec(self, v):
+        psi = np.zeros(self.length, np.complex128)
+        psi[self.allowed] = v
+        psi = psi.reshape((self.d, self.chiL, self.d, self.chiR), order="F")   # (s1, a0, s2, a2)
+        t1 = np.tensordot(self.L, psi, axes=([2], [1]))                         # (a0', b, s1, s2, a2)
+        t2 = np.tensordot(t1, self.W, axes=([1, 2], [1, 2]))                    # (a0', s2, a2, s1', b1)
+        t3 = np.tensordot(t2, self.W, axes=([4, 1], [1, 2]))                    # (a0', a2, s1', s2', b2)
+        o = np.tensordot(t3, self.R, axes=([1, 4], [2, 1]))                     # (a0', s1', s2', a2')
+        o = np.transpose(o, (1, 0, 2, 3))
+        return o.reshape(-1, order="F")[self.allowed]
+
+
 def gauge_fix(vec):
     """Phase gauge of DMRG_routines.cpp:870-890: largest |x| made real
     positive, entries below eps*max dropped, FixComplexPrecision."""

@@ -228,12 +228,14 @@ def sqrt_dr(M: MPX):
     return roots, MPX(M.rules, idx, 1, R)
 
 
-def orthogonalise(mats, lambdas):
+def orthogonalise(mats, lambdas, ncv=NCV):
     """Orthogonalise (measurement.cpp:180-246) of a two-vertex cell.
-    Returns (new matrices, new lambdas, info dict)."""
+    Returns (new matrices, new lambdas, info dict).  ``ncv`` is the Krylov
+    dimension of the fixed-point eigensolves (the reference's 20; a larger
+    one reaches the same fixed point in fewer restarts)."""
     A0, A1 = mats
     rules = A0.rules
-    lamL, ML, _ = transfer_leading(mats, 0)
+    lamL, ML, _ = transfer_leading(mats, 0, ncv)
     if abs(lamL.imag) >= IMAGTOL * abs(lamL.real):
         raise OrthogonaliseError("Eigenvalue has non negligible imaginary part")
     q0 = A0.indices[1].charges
@@ -245,7 +247,7 @@ def orthogonalise(mats, lambdas):
     X = contract(diagonal_mpx(rules, RL.indices[0], RL.indices[0], rootsL), False, RL, False, [(1, 0)])
     Xinv = contract(reorder(RL, True, [1, 0], 1), False,
                     diagonal_mpx(rules, RL.indices[0], RL.indices[0], rootsL, inverse=True), False, [(1, 0)])
-    lamR, MR, _ = transfer_leading(mats, 1)
+    lamR, MR, _ = transfer_leading(mats, 1, ncv)
     if abs(lamR.imag) >= IMAGTOL * abs(lamR.real):
         raise OrthogonaliseError("Eigenvalue has non negligible imaginary part")
     q1 = A1.indices[2].charges

@@ -41,15 +41,26 @@ def even_bond_hamiltonian(W: MPX) -> MPX:
     return reorder(contract(left, False, right, False, [(3, 1)]), False, [0, 3, 2, 4, 1, 5], 2)
 
 
-def bond_evolution_operator(H: MPX, tau: float) -> MPX:
-    """exp(-i tau H) per charge block of the (d^2 x d^2) bond Hamiltonian,
-    indices = the first four of H."""
+def bond_eigensystem(H: MPX):
+    """Per charge block of the bond Hamiltonian: (columns, eigenvalues,
+    eigenvectors) of its Hermitian part (sparse_interface.cpp:1414-1441)."""
     m = H.matrix.toarray()
-    groups = H.get_all_block_columns()
-    U = np.zeros_like(m)
-    for _, cols in groups:
+    out = []
+    for _, cols in H.get_all_block_columns():
         blk = m[np.ix_(cols, cols)]
         e, v = np.linalg.eigh(0.5 * (blk + blk.conj().T))
+        out.append((cols, e, v))
+    return out
+
+
+def bond_evolution_operator(H: MPX, tau: float, eig=None) -> MPX:
+    """exp(-i tau H) per charge block of the (d^2 x d^2) bond Hamiltonian,
+    indices = the first four of H (Exponentiate, sparse_interface.cpp:
+    1707-1714); ``eig`` reuses a bond_eigensystem of H."""
+    eig = bond_eigensystem(H) if eig is None else eig
+    n = H.matrix.shape[0]
+    U = np.zeros((n, n), np.complex128)
+    for cols, e, v in eig:
         U[np.ix_(cols, cols)] = (v * np.exp(-1j * tau * e)) @ v.conj().T
     s = sp.csc_matrix(U)
     s.eliminate_zeros()
@@ -98,8 +109,9 @@ def trotter_operators(W: MPX, tau: float, order: int):
     if order == 1:
         return [bond_evolution_operator(Hodd, tau), bond_evolution_operator(Heven, tau)]
     if order == 2:
-        return [bond_evolution_operator(Hodd, 0.5 * tau), bond_evolution_operator(Heven, tau),
-                bond_evolution_operator(Hodd, tau)]
+        eo = bond_eigensystem(Hodd)
+        return [bond_evolution_operator(Hodd, 0.5 * tau, eo), bond_evolution_operator(Heven, tau),
+                bond_evolution_operator(Hodd, tau, eo)]
     if order == 4:
         from .ftebd import trotter_operators as pieces
         ops, seq = pieces(W, tau, 4)

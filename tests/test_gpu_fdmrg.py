@@ -30,8 +30,15 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 def _compare(dev_steps, ref_steps, tol_e=1e-10):
     assert len(dev_steps) == len(ref_steps)
     for k, (g, r) in enumerate(zip(dev_steps, ref_steps)):
+        print(f"  step {k} ({g['left']},{g['right']}) E {g['energy']:.14f} / {r['energy']:.14f} chi {g['chi']}/"
+              f"{r['chi']} S {g['entropy']:.12f}/{r['entropy']:.12f} tr {g['truncation']:.3e}/{r['truncation']:.3e}"
+              f" fid {g['fidelity']:.3e}/{r['fidelity']:.3e}", flush=True)
         assert (g["left"], g["right"]) == (r["left"], r["right"]), k
-        assert g["chi"] == r["chi"], (k, g["chi"], r["chi"])
+        # without truncation the kept count includes the numerically zero
+        # singular values of the block SVDs, which depend on rounding (the
+        # entropy, compared below, fixes every weight above them)
+        if r["truncation"] > 0:
+            assert g["chi"] == r["chi"], (k, g["chi"], r["chi"])
         assert abs(g["energy"] - r["energy"]) <= tol_e * max(1.0, abs(r["energy"])), (k, g["energy"], r["energy"])
         assert abs(g["entropy"] - r["entropy"]) <= 1e-10, (k, g["entropy"], r["entropy"])
         assert abs(g["truncation"] - r["truncation"]) <= 1e-10, (k, g["truncation"], r["truncation"])

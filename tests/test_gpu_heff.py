@@ -126,11 +126,20 @@ def test_sliced_matvec(ctx, case):
     r, c, ws = H.slices()
     assert ws <= full / 5 and r * c >= 5
     assert rel_err(H.apply(v), ref) <= 1e-12
-    if case == "z2_chain":
-        E, vec, info = H.lowest(converge=True)
-        H.set_slices(1, 1)
-        E1, vec1, _ = H.lowest(converge=True)
-        assert abs(E - E1) <= 1e-10 * max(1.0, abs(E1))
+
+
+def test_sliced_eigensolve(ctx):
+    """The Lanczos eigensolve on a sliced matvec (a Hermitian H_eff from
+    Heisenberg environments) reaches the unsliced lowest eigenvalue."""
+    print("case heff_sliced_eigensolve", flush=True)
+    import chainamps_amd as cam
+    rules, LB, W, RB = CASES["heisenberg"]()
+    H = cam.Heff(to_lib(ctx, LB), to_lib(ctx, W), to_lib(ctx, RB))
+    E1, _, _ = H.lowest(converge=True)
+    H.set_slices(4, 2)
+    for mixed in (False, True):
+        E, vec, info = H.lowest(converge=True, mixed=mixed)
+        assert abs(E - E1) <= 1e-10 * max(1.0, abs(E1)), (mixed, E, E1)
 
 
 def test_heff_instances_share_workspace(ctx):

@@ -66,6 +66,9 @@ def test_matvec_vs_dense(kind):
     out = comp.matvec(v)
     ref = dense_heff_apply(LB, W, RB, comp, v)
     assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+    # the oracle's tensordot restatement (used for large eigensolves) agrees
+    sw = dmrg.StagewiseHeff(LB, W, RB, comp).matvec(v)
+    assert np.linalg.norm(sw - ref) <= 1e-13 * np.linalg.norm(ref)
 
 
 def test_allowed_indices_literal_loop():
