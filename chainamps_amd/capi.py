@@ -72,6 +72,7 @@ _SIGS = {
     "cam_heff_apply": [_p, _p, _p, C.c_int, C.c_int],
     "cam_heff_apply_blocked": [_p, _p, _p, C.c_int],
     "cam_heff_apply_blocked32": [_p, _p, _p],
+    "cam_heff_apply_blocked_noalloc": [_p, _p, _p],
     "cam_heff_set_projectors": [_p, C.c_int, C.POINTER(_dp), _dp],
     "cam_ctx_release_workspaces": [_p],
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
@@ -343,6 +344,11 @@ class Heff:
         ptrs = (_dp * max(len(vecs), 1))(*[_ptr(v.view(np.float64), C.c_double) for v in vecs])
         _check(_LIB.cam_heff_set_projectors(self.h, len(vecs), ptrs, _ptr(w, C.c_double) if len(w) else None))
         self._projectors = vecs
+
+    def apply_device_noalloc(self, din, dout):
+        """FP64 matvec on device block-layout vectors with the intermediates
+        in the library's per-device scratch array (no device allocation)."""
+        _check(_LIB.cam_heff_apply_blocked_noalloc(self.h, din, dout))
 
     def apply_device32(self, din, dout):
         """din/dout: device pointers (ints) of n complex64 (split-complex

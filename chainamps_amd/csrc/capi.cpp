@@ -12,6 +12,7 @@
 #include "ftebd.hpp"
 #include "storage.hpp"
 #include "ortho.hpp"
+#include "noalloc.hpp"
 #include <memory>
 
 using namespace cam;
@@ -62,7 +63,16 @@ int cam_ctx_create(int device, cam_ctx** out) {
   int n = 0;
   HIP_CHECK(hipGetDeviceCount(&n));
   CAM_REQUIRE(device >= 0 && device < n, "no such HIP device");
-  *out = new cam_ctx(device);
+  auto* c = new cam_ctx(device);
+  try {
+    // the no-allocation matvec's scratch array on this device (the
+    // context constructor made the device current)
+    noalloc_setup(device);
+  } catch (...) {
+    delete c;
+    throw;
+  }
+  *out = c;
   CAM_API_END
 }
 
@@ -291,6 +301,14 @@ int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int preci
   set_device(h->owner);
   CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
   h->h.apply(reinterpret_cast<const cplx*>(in), reinterpret_cast<cplx*>(out), precision);
+  CAM_API_END
+}
+
+int cam_heff_apply_blocked_noalloc(cam_heff* h, const double* in, double* out) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(h && in && out, "null argument");
+  set_device(h->owner);
+  heff_apply_noalloc(h->h, reinterpret_cast<const cplx*>(in), reinterpret_cast<cplx*>(out));
   CAM_API_END
 }
 

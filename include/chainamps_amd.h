@@ -132,6 +132,20 @@ int cam_heff_apply(cam_heff* h, const double* in, double* out, int mem_kind, int
 /* Same matvec on vectors already in the device block layout (the Lanczos
  * layout), no gather/scatter; `in`/`out` device pointers of n complex. */
 int cam_heff_apply_blocked(cam_heff* h, const double* in, double* out, int precision);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 cam_heff_apply_blocked with the matvec's two
+ * intermediate buffers in a per-device scratch array of
+ * CAM_NOALLOC_SCRATCH_BYTES (256 MiB, held by every GPU the library runs on
+ * until the process ends) instead of the context workspaces.  It calls no
+ * hipMalloc, hipFree or allocating library routine; it fails with an error
+ * when the intermediates exceed the array (slice the matvec with
+ * cam_heff_set_slices first) or when the stage plans are not yet on the
+ * device at their launch configuration (one cam_heff_apply_blocked call, or
+ * cam_heff_set_launch, puts them there).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU race. */
+#define CAM_NOALLOC_SCRATCH_BYTES (256LL << 20)
+int cam_heff_apply_blocked_noalloc(cam_heff* h, const double* in, double* out);
 /* The FP32 matvec on FP32 split-complex vectors in the device block layout
  * (n complex = 2n floats each), as the mixed-precision eigensolver runs it:
  * no FP64 conversion of in or out. */
