@@ -3,21 +3,27 @@
 Workload: the d=64 Z2 coupled-chain effective Hamiltonian (levels E_n
 sorted uniform[0,8], parity n mod 2, Gaussian inter-chain operator between
 opposite parities, J_perp = 0.1, MPO bond dim 3) with environments of bond
-dimension chi split into two parity sectors (default chi = 500, the largest
-chi of the d=64 configurations whose FP64 matvec, Lanczos basis and
-environments fit one 288 GB device; chi = 2000 needs 131 GB per FP64
-vector).  Environments and psi are complex normal on every QN-allowed entry.
+dimension chi split into two parity sectors, complex normal on every
+QN-allowed entry (Hermitian per MPO channel).
 
 A "step" is one H_eff . psi matvec (TwoVertexMPOMPSMultiply,
 DMRG_routines.cpp:898-974) on a device-resident vector in block layout.
-`value` is the FP64 matvec throughput in TFLOP/s of algorithmic flops
-(sum of 8*m*n*k over the QN-allowed block triples of the four-stage order
-L, W, W, R).  The same line carries the FP32 split-complex throughput, the
-roofline of the dominant kernel (HIP events on the launching stream) and
-the CPU baseline (the oracle's reference-order matvec, timed on a bounded
-sample on the host).
+`value` is the FP64 matvec throughput at chi = 2000 (n = 8.19e9, 131 GB
+per FP64 vector) in TFLOP/s of algorithmic flops (sum of 8*m*n*k over the
+QN-allowed block triples of the four-stage order L, W, W, R).  At that size
+the two vectors leave room for about a thirtieth of the intermediates, so
+the matvec runs in slices of the a0' rows and a2 columns; the FP32
+split-complex matvec runs on FP32 vectors with fewer slices.  The same line
+carries the roofline of the dominant kernel (HIP events on the launching
+stream), the chi = 500 section (matvecs, the growth-step eigensolve and
+whole iDMRG growth steps in FP64 and FP32-mixed, the driver's
+orthogonalisation: the largest chi at which a whole iDMRG step fits one
+GPU), the U(1) batching workload (chi = 1024, 21 sectors, d = 32, w = 6)
+and the CPU baseline (the oracle's reference-order matvec on a bounded
+sample, on the host).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 500]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 2000]
+                       [--chi-small 500] [--no-small] [--no-batching] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -31,23 +37,22 @@ import numpy as np
 
 FP64_MFMA_PEAK = 78.6      # TFLOP/s, MI355X dense FP64 matrix
 FP32_MFMA_PEAK = 157.3     # TFLOP/s, MI355X dense FP32 matrix
+HBM_PEAK = 8.0             # TB/s
 HERE = os.path.dirname(os.path.abspath(__file__))
-# HBM traffic of the dominant stage kernel, measured with rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE and calibrated on a plain device copy; the summary
-# (with the commands that produced it) is committed next to the kernel trace
-TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "heff_matvec_chi500_d64.md")
 
 
 def committed_traffic(chi, d):
-    """Bytes per launch of stage L from the committed profile summary (the
-    line `traffic_bytes_per_launch: <value>`), or None for another workload
-    or when the summary is absent."""
-    if (chi, d) != (500, 64) or not os.path.exists(TRAFFIC_PROFILE):
-        return None
+    """HBM bytes per launch of the dominant stage from the committed profile
+    summary profiles/heff_matvec_chi<chi>_d<d>.md (the line
+    `traffic_bytes_per_launch: <value>`, from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE calibrated on a device copy), or None when absent."""
     import re
-    with open(TRAFFIC_PROFILE) as f:
+    path = os.path.join(HERE, "profiles", f"heff_matvec_chi{chi}_d{d}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
         m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
-    return float(m.group(1)) if m else None
+    return (float(m.group(1)) if m else None), os.path.relpath(path, HERE)
 
 
 def copy_calibration(gib=1.0, reps=3):
@@ -63,6 +68,10 @@ def copy_calibration(gib=1.0, reps=3):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": n, "reps": reps, "read_bytes_per_copy": n, "write_bytes_per_copy": n}))
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(budget_s=20.0):
@@ -181,7 +190,7 @@ def idmrg_steps(ctx, cam, workloads, a):
         dm = cam.IDMRG(W, converge=False, mixed=mixed)
         steps = []
         for _ in range(a.idmrg_steps):
-            r = dm.grow(a.chi)
+            r = dm.grow(a.chi_small)
             steps.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
                           if k in ("N", "chi", "n", "energy", "truncation", "seconds", "eig_seconds",
                                    "svd_seconds", "block_seconds", "matvecs_fp64", "matvecs_fp32")})
@@ -198,18 +207,200 @@ def idmrg_steps(ctx, cam, workloads, a):
     return out
 
 
+
+def matvec_section(ctx, cam, workloads, chi, d, steps, warmup):
+    """FP64 and FP32 matvec at (chi, d) on device-resident vectors in the
+    block layout: FP64 with FP64 vectors, then FP32 split complex on FP32
+    vectors (the mixed eigensolver's storage), each sliced as the memory
+    left by its vectors requires; per-stage durations from HIP events on
+    the context stream and the autotuned launches."""
+    import torch
+    out = {}
+    t0 = time.perf_counter()
+    L, W, R = workloads.coupled_chain_heff_inputs(ctx, chi=chi, d=d, seed=0, Jperp=0.1, hermitian=True)
+    n = (chi * d) ** 2 // 2          # two equal parity sectors on every index
+    free0, total = torch.cuda.mem_get_info()
+    out["memory"] = {"device_total_bytes": total, "free_before_vectors_bytes": free0,
+                     "fp64_vector_bytes": 16 * n, "fp32_vector_bytes": 8 * n}
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.randn(n, dtype=torch.complex128, device=dev, generator=g)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    H = cam.Heff(L, W, R)            # slices chosen from the memory the vectors leave
+    assert H.n == n, (H.n, n)
+    flops = H.flops
+    out["n"], out["flops_per_matvec"] = n, flops
+    out["setup_s"] = round(time.perf_counter() - t0, 2)
+    res = {}
+    for prec, name in ((cam.CAM_FP64, "fp64"), (cam.CAM_FP32, "fp32")):
+        if prec == cam.CAM_FP32:
+            # FP32 vectors from the FP64 input; the FP64 buffers and the
+            # FP64-sized workspaces are released and the matvec re-sliced
+            # for FP32 intermediates in the memory left
+            del y
+            ctx.release_workspaces()
+            torch.cuda.empty_cache()
+            x32 = x.to(torch.complex64)
+            del x
+            torch.cuda.empty_cache()
+            y32 = torch.empty_like(x32)
+            free, _ = torch.cuda.mem_get_info()
+            H.set_workspace_limit(free - 6e9, cam.CAM_FP32)
+            xin, yout = x32.data_ptr(), y32.data_ptr()
+            run = lambda: H.apply_device32(xin, yout)
+        else:
+            xin, yout = x.data_ptr(), y.data_ptr()
+            run = lambda: H.apply_device(xin, yout, cam.CAM_FP64)
+        rs, cs, wsb = H.slices()
+        t = time.perf_counter()
+        for _ in range(warmup):       # the first matvec autotunes the stage launches
+            run()
+        ctx.sync()
+        log(f"chi={chi} {name}: slices {rs}x{cs}, warm-up (autotune) {time.perf_counter() - t:.1f} s")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            run()
+        ctx.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        ms, fl = H.profile_device(xin, yout, prec, reps=1)
+        res[name] = {"ms_per_step": round(el / steps * 1e3, 3), "tflops": round(flops * steps / el / 1e12, 4),
+                     "slices": [rs, cs], "workspace_bytes": wsb * (0.5 if prec == cam.CAM_FP32 else 1.0),
+                     "stage_ms": [round(v, 3) for v in ms], "launch": H.get_launch(prec)}
+        log(f"chi={chi} {name}: {res[name]['ms_per_step']} ms per matvec, {res[name]['tflops']} TFLOP/s")
+    out.update(res)
+    out["stage_gflop"] = [round(v / 1e9, 3) for v in fl]
+    out["stage_bytes"] = H.stage_bytes()
+    out["speedup_fp32_vs_fp64"] = round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3)
+    del x32, y32
+    H.close()
+    for t_ in (L, W, R):
+        t_.close()
+    ctx.release_workspaces()
+    torch.cuda.empty_cache()
+    return out
+
+
+def roofline(sec, chi, d):
+    """The dominant stage kernel (stage L, by time) against the FP64 MFMA
+    peak: its algorithmic flops per launch over its average launch duration
+    (HIP events); with s slices a launch is one slice's share."""
+    ms = sec["fp64"]["stage_ms"]
+    dom = int(np.argmax(ms))
+    rs, cs = sec["fp64"]["slices"]
+    ns = rs * cs
+    achieved = sec["stage_gflop"][dom] * 1e9 / (ms[dom] * 1e-3) / 1e12
+    traffic, src = committed_traffic(chi, d)
+    names = ["L", "W1", "W2", "R"]
+    return {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_MFMA_PEAK, 4),
+            "traffic": traffic if names[dom] == "L" else None,
+            "traffic_source": src,
+            "algorithmic_bytes": sec["stage_bytes"][dom] / ns,
+            "launch_ms": round(ms[dom] / ns, 4), "launches_per_matvec": ns,
+            "kernel": f"grouped_gemm_kernel<double> stage {names[dom]} ({rs}x{cs} slices)"}
+
+
+def eigensolve_section(ctx, cam, workloads, chi, d):
+    """The growth-step eigensolve (the reference's setting: tol 0.1, one
+    Krylov cycle of ncv = 20, arpack_interface.cpp:28-39) on the Hermitian
+    environments at (chi, d), FP64 against FP32-mixed, same start vector."""
+    import torch
+    L, W, R = workloads.coupled_chain_heff_inputs(ctx, chi=chi, d=d, seed=0, Jperp=0.1, hermitian=True)
+    H = cam.Heff(L, W, R)
+    n = H.n
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.randn(n, dtype=torch.complex128, device=dev, generator=g)
+    y = torch.empty_like(x)
+    eig = {}
+    for mixed, name in ((False, "fp64"), (True, "mixed")):
+        y.copy_(x)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        E, info = H.lowest_device(y.data_ptr(), have_start=True, converge=False, ncv=20, mixed=mixed)
+        el = time.perf_counter() - t
+        eig[name] = dict(seconds=round(el, 3), energy=E, matvecs_fp64=info["matvecs_fp64"],
+                         matvecs_fp32=info["matvecs_fp32"], residual=info["residual"])
+        log(f"chi={chi} growth-step eigensolve {name} {el:.2f} s")
+    eig["speedup_mixed_vs_fp64"] = round(eig["fp64"]["seconds"] / eig["mixed"]["seconds"], 3)
+    eig["note"] = ("Lanczos eigensolve with the growth-step setting; the SVD and the block updates of the step "
+                   "are not included")
+    del x, y
+    H.close()
+    ctx.release_workspaces()
+    torch.cuda.empty_cache()
+    return eig
+
+
+def batching_section(ctx, cam, workloads, steps, warmup):
+    """The U(1) batching workload at full size: chi = 1024 in 21 sectors
+    (charges -10..10, Gaussian sizes), d = 32 with charges in [-3, 3],
+    w = 6, complex normal entries (seed 0); FP64 and FP32 matvecs on
+    device block-layout vectors with the autotuned launches."""
+    import torch
+    L, W, R = workloads.u1_batching_inputs(ctx, chi=1024)
+    H = cam.Heff(L, W, R)
+    n, flops = H.n, H.flops
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    x = torch.randn(n, dtype=torch.complex128, device=dev, generator=g)
+    y = torch.empty_like(x)
+    x32 = x.to(torch.complex64)
+    y32 = torch.empty_like(x32)
+    out = {"workload": "U(1) batching: chi=1024 in 21 sectors (charges -10..10, Gaussian sizes sigma=4), d=32 "
+                       "(charges uniform in [-3,3]), w=6, complex normal entries, seed 0",
+           "n": n, "flops_per_matvec": flops}
+    reps = max(steps, 20)
+    for prec, name in ((cam.CAM_FP64, "fp64"), (cam.CAM_FP32, "fp32")):
+        run = ((lambda: H.apply_device(x.data_ptr(), y.data_ptr(), cam.CAM_FP64)) if prec == cam.CAM_FP64
+               else (lambda: H.apply_device32(x32.data_ptr(), y32.data_ptr())))
+        for _ in range(max(warmup, 1)):
+            run()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            run()
+        ctx.sync()
+        el = time.perf_counter() - t
+        ms, fl = (H.profile_device(x.data_ptr(), y.data_ptr(), prec, reps=3) if prec == cam.CAM_FP64 else
+                  H.profile_device(x32.data_ptr(), y32.data_ptr(), prec, reps=3))
+        peak = FP64_MFMA_PEAK if prec == cam.CAM_FP64 else FP32_MFMA_PEAK
+        out[name] = {"ms_per_step": round(el / reps * 1e3, 4), "tflops": round(flops * reps / el / 1e12, 4),
+                     "frac_of_peak": round(flops * reps / el / 1e12 / peak, 4),
+                     "stage_ms": [round(v, 4) for v in ms], "launch": H.get_launch(prec)}
+        log(f"batching {name}: {out[name]['ms_per_step']} ms, {out[name]['tflops']} TFLOP/s")
+    out["stage_gflop"] = [round(v / 1e9, 3) for v in fl]
+    out["speedup_fp32_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
+    out["steps"] = reps
+    del x, y, x32, y32
+    H.close()
+    ctx.release_workspaces()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--chi", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--chi", type=int, default=2000, help="bond dimension of the headline matvec")
+    ap.add_argument("--chi-small", type=int, default=500,
+                    help="bond dimension of the eigensolve / iDMRG step section (whole steps fit one GPU)")
     ap.add_argument("--d", type=int, default=64)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the chi-small section")
     ap.add_argument("--no-eig", action="store_true", help="skip the growth-step eigensolve timing")
     ap.add_argument("--no-idmrg", action="store_true", help="skip the iDMRG step timing")
+    ap.add_argument("--no-batching", action="store_true", help="skip the U(1) batching workload")
+    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--idmrg-steps", type=int, default=4,
-                    help="growth steps per precision (the bond reaches chi at step 3 for d=64)")
+                    help="growth steps per precision (the bond reaches chi-small at step 2 for d=64)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate the HBM byte counters")
@@ -227,116 +418,56 @@ def main():
     from chainamps_amd import workloads
 
     ctx = cam.Context(0)
-    t0 = time.perf_counter()
-    L, W, R = workloads.coupled_chain_heff_inputs(ctx, chi=a.chi, d=a.d, seed=0, Jperp=0.1, hermitian=True)
-    H = cam.Heff(L, W, R)
-    setup_s = time.perf_counter() - t0
-    n = H.n
-    flops = H.flops
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(0)
-    x = torch.randn(n, dtype=torch.complex128, device=dev, generator=g)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-
-    # FP32 vectors (split complex) for the FP32 matvec as the mixed solver
-    # runs it; the FP64-vector variant (conversion in and out) is kept too
-    x32 = x.to(torch.complex64)
-    y32 = torch.empty_like(x32)
-
-    def run(prec, k):
-        for _ in range(k):
-            if prec == "fp32":
-                H.apply_device32(x32.data_ptr(), y32.data_ptr())
-            else:
-                H.apply_device(x.data_ptr(), y.data_ptr(), prec)
-        ctx.sync()
-
-    res = {}
-    for prec, name in ((cam.CAM_FP64, "fp64"), ("fp32", "fp32"), (cam.CAM_FP32, "fp32_fp64_vectors")):
-        run(prec, a.warmup)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        run(prec, a.steps)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t
-        res[name] = dict(ms=el / a.steps * 1e3, tflops=flops * a.steps / el / 1e12)
-    del x32, y32
-    # roofline of the dominant kernel, from HIP events on the context stream
-    ms64, fl = H.profile(cam.CAM_FP64, reps=max(3, min(a.steps, 10)))
-    ms32, _ = H.profile(cam.CAM_FP32, reps=max(3, min(a.steps, 10)))
-    dom = int(np.argmax(ms64))
-    achieved = fl[dom] / (ms64[dom] * 1e-3) / 1e12
-    stage_names = ["L", "W1", "W2", "R"]
-    stage_bytes = H.stage_bytes()
-    # eigensolve of one iDMRG growth step (the reference's setting for it:
-    # converge=0, i.e. tol 0.1 and one Krylov cycle of ncv=20 vectors,
-    # arpack_interface.cpp:28-39) from the same start vector, FP64 against
-    # FP32-mixed (FP64 first step and reorthogonalisation, FP32 matvecs)
-    eig = {}
-    free, _ = torch.cuda.mem_get_info()
-    need = 23 * 16 * n          # Krylov basis (ncv+1 vectors) and work vector, FP64
-    if free < 1.05 * need:
-        eig = {"skipped": f"needs {need / 1e9:.0f} GB of free device memory, {free / 1e9:.0f} GB free"}
-    elif not a.no_eig:
-        for mixed, name in ((False, "fp64"), (True, "mixed")):
-            y.copy_(x)
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            E, info = H.lowest_device(y.data_ptr(), have_start=True, converge=False, ncv=20, mixed=mixed)
-            el = time.perf_counter() - t
-            eig[name] = dict(seconds=round(el, 3), energy=E, matvecs_fp64=info["matvecs_fp64"],
-                             matvecs_fp32=info["matvecs_fp32"], residual=info["residual"])
-        eig["speedup_mixed_vs_fp64"] = round(eig["fp64"]["seconds"] / eig["mixed"]["seconds"], 3)
-        eig["note"] = ("Lanczos eigensolve with the growth-step setting; the SVD and the block updates of "
-                       "the step are not included")
-    launch = {"fp64": H.get_launch(cam.CAM_FP64), "fp32": H.get_launch(cam.CAM_FP32)}
-    # whole iDMRG growth steps (SuperBlock::grow_two_vertex, DMRG_routines.cpp:309-374)
-    # of the same model MPO: block updates, H_eff build, eigensolve from the
-    # predicted guess (growth setting), SVD truncation to chi and prediction,
-    # FP64 against FP32-mixed; the matvec buffers above are released first
-    del x, y
-    H.close()
-    torch.cuda.empty_cache()
-    idmrg = None if a.no_idmrg else idmrg_steps(ctx, cam, workloads, a)
+    head = matvec_section(ctx, cam, workloads, a.chi, a.d, a.steps, a.warmup)
+    small = None
+    if not a.no_small:
+        small = matvec_section(ctx, cam, workloads, a.chi_small, a.d, max(a.steps, 10), a.warmup)
+        small["roofline"] = roofline(small, a.chi_small, a.d)
+        if not a.no_eig:
+            small["growth_step_eigensolve"] = eigensolve_section(ctx, cam, workloads, a.chi_small, a.d)
+        if not a.no_idmrg:
+            small["idmrg_step"] = idmrg_steps(ctx, cam, workloads, a)
+    batching = None if a.no_batching else batching_section(ctx, cam, workloads, a.steps, a.warmup)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_budget)
+    fp64, fp32 = head["fp64"], head["fp32"]
+    names = ["L", "W1", "W2", "R"]
     line = {
         "metric": "H_eff matvec TFLOP/s + iDMRG step time, chi=2000 d=64, FP32-mixed vs FP64",
-        "value": round(res["fp64"]["tflops"], 4),
+        "value": fp64["tflops"],
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(res["fp64"]["ms"], 4),
+        "ms_per_step": fp64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp64 (complex128)",
         "data": "synthetic (seeded QN-allowed complex normal environments and psi; model MPO)",
-        "config": {"workload": f"two-vertex H_eff matvec, d={a.d} Z2 coupled chain, w=3, "
-                               f"chi={a.chi} (2 parity sectors)",
-                   "chi": a.chi, "d": a.d, "mpo_bond": 3, "n": n, "flops_per_matvec": flops,
-                   "parallelism": "single GPU"},
-        "fp32_split": {"tflops": round(res["fp32"]["tflops"], 4), "ms_per_step": round(res["fp32"]["ms"], 4),
-                       "speedup_vs_fp64": round(res["fp64"]["ms"] / res["fp32"]["ms"], 3),
+        "config": {"workload": f"two-vertex H_eff matvec, d={a.d} Z2 coupled chain, w=3, chi={a.chi} "
+                               f"(2 parity sectors), vectors resident in HBM",
+                   "chi": a.chi, "d": a.d, "mpo_bond": 3, "n": head["n"],
+                   "flops_per_matvec": head["flops_per_matvec"], "parallelism": "single GPU"},
+        "fp32_split": {"tflops": fp32["tflops"], "ms_per_step": fp32["ms_per_step"],
+                       "speedup_vs_fp64": head["speedup_fp32_vs_fp64"],
                        "vectors": "FP32 split complex in the block layout, as the mixed-precision eigensolver "
-                                  "keeps its Krylov vectors",
-                       "ms_with_fp64_vectors": round(res["fp32_fp64_vectors"]["ms"], 4)},
-        "stages": {stage_names[i]: {"fp64_ms": round(ms64[i], 4), "fp32_ms": round(ms32[i], 4),
-                                    "gflop": round(fl[i] / 1e9, 3)} for i in range(4)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4),
-                     "traffic": committed_traffic(a.chi, a.d) if stage_names[dom] == "L" else None,
-                     "traffic_source": "profiles/heff_matvec_chi500_d64.md (stage L, rocprofv3 --pmc)",
-                     "algorithmic_bytes": stage_bytes[dom],
-                     "kernel": f"grouped_gemm_kernel<double> stage {stage_names[dom]}"},
+                                  "keeps its Krylov vectors"},
+        "slices": {"fp64": fp64["slices"], "fp32": fp32["slices"],
+                   "workspace_bytes": {"fp64": fp64["workspace_bytes"], "fp32": fp32["workspace_bytes"]},
+                   "note": "a0' row x a2 column slices of the intermediates, chosen from the memory the vectors leave"},
+        "stages": {names[i]: {"fp64_ms": fp64["stage_ms"][i], "fp32_ms": fp32["stage_ms"][i],
+                              "gflop": head["stage_gflop"][i]} for i in range(4)},
+        "roofline": roofline(head, a.chi, a.d),
         "cpu_baseline": cpu,
-        "growth_step_eigensolve": eig or None,
-        "idmrg_step": idmrg,
+        "memory": head["memory"],
         "autotuned_launch": {"fields": "(algo 0=4M/1=3M, tile_mode, wg_per_cu, kchunk) per stage L, W1, W2, R",
-                             **launch},
-        "setup_s": round(setup_s, 2),
+                             "fp64": fp64["launch"], "fp32": fp32["launch"]},
+        "idmrg_step_at_chi": ("a whole iDMRG step at chi=2000 does not fit one GPU: its eigensolve needs a "
+                              "Krylov basis of ncv+1 vectors of 131 GB (FP64) or 65.5 GB (FP32); the step is "
+                              "timed at chi_small below"),
+        "chi_small": small,
+        "batching": batching,
+        "setup_s": head["setup_s"],
     }
     print(json.dumps(line))
 

@@ -78,7 +78,7 @@ _SIGS = {
     "cam_heff_profile": [_p, C.c_int, C.c_int, _dp, _dp],
     "cam_heff_profile_device": [_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp],
     "cam_heff_set_slices": [_p, C.c_int, C.c_int],
-    "cam_heff_set_workspace_limit": [_p, C.c_double],
+    "cam_heff_set_workspace_limit": [_p, C.c_double, C.c_int],
     "cam_heff_get_slices": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp],
     "cam_heff_set_launch": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "cam_heff_get_launch": [_p, C.c_int, C.POINTER(C.c_int)],
@@ -376,9 +376,9 @@ class Heff:
         intermediates (1 x 1: unsliced)."""
         _check(_LIB.cam_heff_set_slices(self.h, int(row_slices), int(col_slices)))
 
-    def set_workspace_limit(self, nbytes):
-        """Fewest slices whose intermediates fit nbytes."""
-        _check(_LIB.cam_heff_set_workspace_limit(self.h, float(nbytes)))
+    def set_workspace_limit(self, nbytes, precision=CAM_FP64):
+        """Fewest slices whose intermediates fit nbytes at `precision`."""
+        _check(_LIB.cam_heff_set_workspace_limit(self.h, float(nbytes), int(precision)))
 
     def slices(self):
         """(row slices, column slices, workspace bytes)."""

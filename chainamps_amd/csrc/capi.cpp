@@ -352,11 +352,13 @@ int cam_heff_set_slices(cam_heff* h, int row_slices, int col_slices) {
   CAM_API_END
 }
 
-int cam_heff_set_workspace_limit(cam_heff* h, double bytes) {
+int cam_heff_set_workspace_limit(cam_heff* h, double bytes, int precision) {
   CAM_API_BEGIN
   set_device(h->owner);
   CAM_REQUIRE(bytes > 0.0, "workspace limit must be positive");
-  h->h.choose_slices(bytes);
+  CAM_REQUIRE(precision == CAM_FP64 || precision == CAM_FP32, "bad precision");
+  // FP32 intermediates take 8 bytes per element, the FP64 count's half
+  h->h.choose_slices(precision == CAM_FP32 ? 2.0 * bytes : bytes);
   h->h.plan_slices(h->h.rslices, h->h.cslices);
   HIP_CHECK(hipStreamSynchronize(h->owner->ctx.stream));
   CAM_API_END

@@ -194,7 +194,7 @@ void Heff::build(Ctx& c, TensorPtr L_, TensorPtr W_, TensorPtr R_, double worksp
     size_t fr = 0, tot = 0;
     HIP_CHECK(hipMemGetInfo(&fr, &tot));
     const double have = double(fr) + sizeof(cplx) * double(ctx->ws_t1.size() + ctx->ws_t2.size());
-    workspace_limit = have - 4.0e9;
+    workspace_limit = have - 6.0e9;   // the plans' tile lists, library scratch
   }
   choose_slices(workspace_limit);
   plan_slices(rslices, cslices);
@@ -371,19 +371,22 @@ void Heff::plan_slices(int rs, int cs) {
         flops += st[i].flops;
       }
     }
-  bind_workspace();
 }
 
-void Heff::bind_workspace() {
-  ctx->ws_t1.ensure(std::max<int64_t>(ws1n, 1));
-  ctx->ws_t2.ensure(std::max<int64_t>(ws2n, 1));
+// The intermediates of an FP32 matvec take 8 bytes per element, half the
+// FP64 size; the context workspaces are grow-only, so they are sized at the
+// first use of a precision (an FP32-only caller never holds the FP64 size).
+void Heff::bind_workspace(int precision) {
+  const int64_t e1 = precision ? (ws1n + 1) / 2 : ws1n, e2 = precision ? (ws2n + 1) / 2 : ws2n;
+  ctx->ws_t1.ensure(std::max<int64_t>(e1, 1));
+  ctx->ws_t2.ensure(std::max<int64_t>(e2, 1));
   T1.p = ctx->ws_t1.get();
   T2.p = ctx->ws_t2.get();
 }
 
 GemmBases Heff::bases(int precision, const void* in, void* out) {
   // another H_eff on this context may have grown the shared workspaces
-  bind_workspace();
+  bind_workspace(precision);
   GemmBases b{};
   if (precision == 0) {
     b.p[kSpL] = L->data.get();

@@ -105,7 +105,7 @@ struct Heff {
   void prepare_fp32();
   // operand bases of a launch (re-binds T1/T2 to the context workspaces)
   GemmBases bases(int precision, const void* in, void* out);
-  void bind_workspace();
+  void bind_workspace(int precision = 0);
   // device block-layout vectors
   void apply_blocked(const cplx* in, cplx* out);
   // the four stage launches (autotuning them on first use outside capture)

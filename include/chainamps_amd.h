@@ -170,10 +170,12 @@ int cam_heff_profile_device(cam_heff* h, int precision, int reps, const void* in
  * intermediate buffers fit the free device memory (plus the context's
  * existing workspaces, less 4 GB): 1 x 1 wherever the whole intermediates
  * fit.  set_slices fixes the counts; set_workspace_limit takes the fewest
- * (powers of two) whose buffers fit `bytes`; get_slices reports the counts
- * and the workspace bytes.  The adjoint needs 1 x 1. */
+ * (powers of two) whose buffers fit `bytes` at `precision` (FP32
+ * intermediates take half the FP64 bytes); get_slices reports the counts
+ * and the FP64 workspace bytes.  The workspaces are sized at the first
+ * matvec of each precision.  The adjoint needs 1 x 1. */
 int cam_heff_set_slices(cam_heff* h, int row_slices, int col_slices);
-int cam_heff_set_workspace_limit(cam_heff* h, double bytes);
+int cam_heff_set_workspace_limit(cam_heff* h, double bytes, int precision);
 int cam_heff_get_slices(const cam_heff* h, int* row_slices, int* col_slices, double* workspace_bytes);
 /* Algorithmic HBM bytes per stage (L, W, W, R) of the FP64 matvec: every
  * operand read once and the output written once (psi -> T1 -> T2 -> T3 ->
