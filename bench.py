@@ -41,17 +41,21 @@ HBM_PEAK = 8.0             # TB/s
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def committed_traffic(chi, d):
-    """HBM bytes per launch of the dominant stage from the committed profile
+def committed_traffic(chi, d, stage):
+    """HBM bytes per launch of a stage kernel from the committed profile
     summary profiles/heff_matvec_chi<chi>_d<d>.md (the line
-    `traffic_bytes_per_launch: <value>`, from rocprofv3 --pmc FETCH_SIZE and
+    `traffic_bytes_per_launch_<stage>: <value>`, or for stage L
+    `traffic_bytes_per_launch: <value>`; from rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE calibrated on a device copy), or None when absent."""
     import re
     path = os.path.join(HERE, "profiles", f"heff_matvec_chi{chi}_d{d}.md")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
-        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        txt = f.read()
+    m = re.search(rf"^traffic_bytes_per_launch_{stage}:\s*([0-9.eE+]+)", txt, re.M)
+    if not m and stage == "L":
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
     return (float(m.group(1)) if m else None), os.path.relpath(path, HERE)
 
 
@@ -293,11 +297,11 @@ def roofline(sec, chi, d):
     rs, cs = sec["fp64"]["slices"]
     ns = rs * cs
     achieved = sec["stage_gflop"][dom] * 1e9 / (ms[dom] * 1e-3) / 1e12
-    traffic, src = committed_traffic(chi, d)
     names = ["L", "W1", "W2", "R"]
+    traffic, src = committed_traffic(chi, d, names[dom])
     return {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_MFMA_PEAK, 4),
-            "traffic": traffic if names[dom] == "L" else None,
+            "traffic": traffic,
             "traffic_source": src,
             "algorithmic_bytes": sec["stage_bytes"][dom] / ns,
             "launch_ms": round(ms[dom] / ns, 4), "launches_per_matvec": ns,
