@@ -158,47 +158,94 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
     int ck = 0;
     GemmSeg CS;
     if (cs < send) CS = cload(segs + cs);
-    auto advance = [&]() {
-      ck += KC;
-      if (ck >= CS.K) {
-        ck = 0;
-        ++cs;
-        if (cs < send) CS = cload(segs + cs);
-      }
-      return cs < send;
-    };
 
-    // operand loads: global address space (not flat, whose completion the
-    // LDS waits of the MFMA loop would also have to wait for) and branch-free:
-    // a padding element reads the block's first element and is zeroed when
-    // it is staged to LDS, so nothing waits on the loads before the MFMAs
-    auto load = [&](const GemmSeg& G, int kk0, C(&ra)[PA], C(&rb)[PB], uint32_t& amask, uint32_t& bmask) {
+    // Address state of the current segment, set up once per segment: the
+    // operand pointers at this tile's batch, first row / column and the
+    // segment's first k, every element's offset from them (32 bits: a tile
+    // spans at most 128 rows or columns and the plans keep strides below
+    // 2^31 / 128), the element stride along k and which elements lie inside
+    // the problem's rows / columns.  A chunk's addresses are then one add per
+    // element (the chunk's k offset is uniform).
+    const C* segA = nullptr;
+    const C* segB = nullptr;
+    int32_t aoff[PA], boff[PB];
+    int64_t sak = 0, sbk = 0;
+    uint32_t arow = 0, bcol = 0;
+    bool akc = false, bnc = false;
+    auto setup = [&](const GemmSeg& G) {
       const int asp = (G.flags >> 8) & 15, bsp = (G.flags >> 12) & 15;
-      const C* A = (asp ? reinterpret_cast<const C*>(bases.p[asp]) + reinterpret_cast<intptr_t>(G.A)
-                        : reinterpret_cast<const C*>(G.A)) +
-                   static_cast<int64_t>(tl.batch) * G.bsA;
-      const C* B = (bsp ? reinterpret_cast<const C*>(bases.p[bsp]) + reinterpret_cast<intptr_t>(G.B)
-                        : reinterpret_cast<const C*>(G.B)) +
-                   static_cast<int64_t>(tl.batch) * G.bsB;
-      const bool akc = G.flags & kSegAKContig, bnc = G.flags & kSegBNContig;
+      akc = G.flags & kSegAKContig;
+      bnc = G.flags & kSegBNContig;
+      const int32_t csA = static_cast<int32_t>(G.csA), ldB = static_cast<int32_t>(G.ldB);
+      segA = (asp ? reinterpret_cast<const C*>(bases.p[asp]) + reinterpret_cast<intptr_t>(G.A)
+                  : reinterpret_cast<const C*>(G.A)) +
+             static_cast<int64_t>(tl.batch) * G.bsA + (akc ? static_cast<int64_t>(m0) * G.csA : m0);
+      segB = (bsp ? reinterpret_cast<const C*>(bases.p[bsp]) + reinterpret_cast<intptr_t>(G.B)
+                  : reinterpret_cast<const C*>(G.B)) +
+             static_cast<int64_t>(tl.batch) * G.bsB + (bnc ? n0 : static_cast<int64_t>(n0) * G.ldB);
+      sak = akc ? 1 : G.csA;
+      sbk = bnc ? G.ldB : 1;
+      arow = 0;
+      bcol = 0;
 #pragma unroll
       for (int p = 0; p < PA; ++p) {
         const int e = tid + p * 256;
         const int m = akc ? e / KC : e % TM, k = akc ? e % KC : e / TM;
-        const int gm = m0 + m, gk = kk0 + k;
-        const bool ok = gm < M && gk < G.K;
-        const int64_t off = akc ? gk + static_cast<int64_t>(gm) * G.csA : gm + static_cast<int64_t>(gk) * G.csA;
-        ra[p] = gload(A + (ok ? off : 0));
-        amask = ok ? (amask | (1u << p)) : (amask & ~(1u << p));
+        aoff[p] = akc ? k + m * csA : m + k * csA;
+        arow |= (m0 + m < M ? 1u : 0u) << p;
       }
 #pragma unroll
       for (int p = 0; p < PB; ++p) {
         const int e = tid + p * 256;
         const int n = bnc ? e % TN : e / KC, k = bnc ? e / TN : e % KC;
-        const int gn = n0 + n, gk = kk0 + k;
-        const bool ok = gn < N && gk < G.K;
-        const int64_t off = bnc ? static_cast<int64_t>(gk) * G.ldB + gn : gk + static_cast<int64_t>(gn) * G.ldB;
-        rb[p] = gload(B + (ok ? off : 0));
+        boff[p] = bnc ? k * ldB + n : k + n * ldB;
+        bcol |= (n0 + n < N ? 1u : 0u) << p;
+      }
+    };
+    if (cs < send) setup(CS);
+    auto advance = [&]() {
+      ck += KC;
+      if (ck >= CS.K) {
+        ck = 0;
+        ++cs;
+        if (cs < send) {
+          CS = cload(segs + cs);
+          setup(CS);
+        }
+      }
+      return cs < send;
+    };
+
+    // operand loads of the chunk at k offset kk0 of the current segment:
+    // global address space (not flat, whose completion the LDS waits of the
+    // MFMA loop would also have to wait for) and branch-free: an element
+    // outside the problem reads the segment's first element and is zeroed
+    // when it is staged to LDS, so nothing waits on the loads before the
+    // MFMAs
+    auto load = [&](int kk0, int K, C(&ra)[PA], C(&rb)[PB], uint32_t& amask, uint32_t& bmask) {
+      const int64_t da = static_cast<int64_t>(kk0) * sak, db = static_cast<int64_t>(kk0) * sbk;
+      const int kleft = K - kk0;
+#pragma unroll
+      for (int p = 0; p < PA; ++p) {
+        const int e = tid + p * 256;
+        const int k = akc ? e % KC : e / TM;
+        const bool ok = ((arow >> p) & 1u) && k < kleft;
+        // a padding element reads the tile's first element of the segment,
+        // inside the operand's allocation; the empty asm keeps the compiler
+        // from turning the select into two loads under divergent branches
+        int64_t o = ok ? aoff[p] + da : 0;
+        asm("" : "+v"(o));
+        ra[p] = gload(segA + o);
+        amask = ok ? (amask | (1u << p)) : (amask & ~(1u << p));
+      }
+#pragma unroll
+      for (int p = 0; p < PB; ++p) {
+        const int e = tid + p * 256;
+        const int k = bnc ? e / TN : e % KC;
+        const bool ok = ((bcol >> p) & 1u) && k < kleft;
+        int64_t o = ok ? boff[p] + db : 0;
+        asm("" : "+v"(o));
+        rb[p] = gload(segB + o);
         bmask = ok ? (bmask | (1u << p)) : (bmask & ~(1u << p));
       }
     };
@@ -336,7 +383,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
     int buf = 0;
     if (cs < send) {
       GemmSeg S0 = CS;
-      load(S0, ck, ra0, rb0, am0, bm0);
+      load(ck, S0.K, ra0, rb0, am0, bm0);
       store(S0, 0, ra0, rb0, am0, bm0);
       __syncthreads();
       // chunk c+1 is loaded into registers while chunk c is multiplied from
@@ -346,7 +393,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
       for (;;) {
         const bool v1 = advance();
         const GemmSeg S1 = CS;
-        load(S1, ck, ra0, rb0, am0, bm0);
+        load(ck, S1.K, ra0, rb0, am0, bm0);
         // keep every load of the next chunk ahead of the MFMAs (the scheduler
         // would otherwise sink some of them next to their LDS stores, where
         // their whole latency is exposed)
