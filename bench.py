@@ -19,8 +19,9 @@ stream), the chi = 500 section (matvecs, the growth-step eigensolve and
 whole iDMRG growth steps in FP64 and FP32-mixed, the driver's
 orthogonalisation: the largest chi at which a whole iDMRG step fits one
 GPU), the U(1) batching workload (chi = 1024, 21 sectors, d = 32, w = 6)
-and the CPU baseline (the oracle's reference-order matvec on a bounded
-sample, on the host).
+and the CPU baseline (oracle/refmatvec, the oracle's C++ reference-order
+matvec, on bounded samples on one host thread and on all of them, with a
+chi-scaling fit).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 2000]
                        [--chi-small 500] [--no-small] [--no-batching] [--no-cpu]
@@ -78,82 +79,91 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(budget_s=20.0):
-    """Oracle restatement of the reference-order matvec (LeftPart . V, then
-    . RightPart via sparse products; DMRG_routines.cpp:779-974) on a reduced
-    chi of the same d=64 model, single host thread.  Reported as
-    algorithmic (optimal-order) TFLOP/s of the sample."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import models, dmrg
-    rules, phys, W, E, sig = models.coupled_chain_model(d=64, seed=0, Jperp=0.1)
-    rng = np.random.default_rng(1)
-    chi = 24
-    aq = (np.arange(chi) % 2).reshape(-1, 1)
-    bq = W.indices[1].charges
-    LB = models.random_left_block(rules, aq, bq, rng)
-    RB = models.random_right_block(rules, aq, bq, rng)
-    comp = dmrg.TwoVertexComponents(LB, W, RB)
-    v = rng.standard_normal(comp.n) + 1j * rng.standard_normal(comp.n)
-    comp.matvec(v)                         # warm
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        comp.matvec(v)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or reps >= 50:
-            break
-    flops = optimal_order_flops_dense(LB, W, RB, comp)
-    return dict(value=flops * reps / el / 1e12, unit="TFLOP/s", cores=1, kind="port",
-                sample=f"oracle reference-order matvec (scipy CSC SpGEMM), d=64 Z2 coupled chain, "
-                       f"chi={chi}, n={comp.n}, {reps} matvecs in {el:.1f}s; flops counted in the "
-                       f"optimal four-stage order ({flops:.3e}/matvec)")
+def matvec_flops_z2(chi, d):
+    """Algorithmic flops of the four-stage matvec on the Z2 coupled chain
+    with two bond sectors of chi/2 and d/2 levels per parity: 8 psi blocks x
+    3 MPO channels x 8 (chi/2)^3 (d/2)^2 in each of stages L and R, and 40
+    (T block, W block) pairs x 8 (chi/2)^2 (d/2)^3 in each W stage."""
+    return 12.0 * chi ** 3 * d ** 2 + 20.0 * chi ** 2 * d ** 3
 
 
-def optimal_order_flops_dense(LB, W, RB, comp):
-    """Sum of 8*m*n*k over nonzero QN block triples of the four stages,
-    counted from the dense tensors (independent of the device planner)."""
-    L = LB.to_dense_tensor()[0, :, 0, :, 0, :]
-    R = RB.to_dense_tensor()[:, 0, :, 0, :, 0]
-    Wd = W.to_dense_tensor()
-    aq = LB.indices[5].charges[:, 0]
-    a2q = RB.indices[4].charges[:, 0]
-    sq = W.indices[2].charges[:, 0]
-    mod = W.rules.rules[0] if W.rules.rules[0] > 0 else None
-    add = (lambda a, b: (a + b) % mod) if mod else (lambda a, b: a + b)
-    secs = lambda q: {c: int(np.sum(q == c)) for c in np.unique(q)}
-    A, A2, S = secs(aq), secs(a2q), secs(sq)
-    w = Wd.shape[1]
-    nzL = {(b, c0): c0p for b in range(w) for c0 in A for c0p in A
-           if np.any(L[np.ix_(aq == c0p, [b], aq == c0)])}
-    nzR = {(b, c3): c3p for b in range(w) for c3 in A2 for c3p in A2
-           if np.any(R[np.ix_(a2q == c3p, [b], a2q == c3)])}
-    nzW = {(bl, br, sp, s_) for bl in range(w) for br in range(w) for sp in S for s_ in S
-           if np.any(Wd[np.ix_(sq == sp, [bl], sq == s_, [br])])}
-    flops = 0.0
-    psi = [(s1, s0, s2, add(add(s1, s0), s2)) for s1 in S for s0 in A for s2 in S if add(add(s1, s0), s2) in A2]
-    T1 = set()
-    for (s1, s0, s2, s3) in psi:
-        for b in range(w):
-            if (b, s0) in nzL:
-                s0p = nzL[(b, s0)]
-                flops += 8 * A[s0p] * A[s0] * A2[s3] * S[s1] * S[s2]
-                T1.add((b, s0p, s3, s1, s2))
-    T2 = set()
-    for (b, s0p, s3, s1, s2) in T1:
-        for (bl, br, sp, s_) in nzW:
-            if bl == b and s_ == s1:
-                flops += 8 * A[s0p] * A2[s3] * S[sp] * S[s1] * S[s2]
-                T2.add((br, s0p, s3, sp, s2))
-    T3 = set()
-    for (b, s0p, s3, s1p, s2) in T2:
-        for (bl, br, sp, s_) in nzW:
-            if bl == b and s_ == s2:
-                flops += 8 * A[s0p] * A2[s3] * S[s1p] * S[sp] * S[s2]
-                T3.add((br, s0p, s3, s1p, sp))
-    for (b, s0p, s3, s1p, s2p) in T3:
-        if (b, s3) in nzR:
-            flops += 8 * A[s0p] * A2[nzR[(b, s3)]] * A2[s3] * S[s1p] * S[s2p]
-    return flops
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=20.0, d=64, targets=()):
+    """The oracle's C++ restatement of the reference-order matvec
+    (oracle/refmatvec.cpp: LeftPart . V, then . RightPart as Gustavson CSC
+    products with the reference's column chunking over threads;
+    DMRG_routines.cpp:898-974, sparse_interface.cpp:1892-2170) on the same
+    d=64 Z2 model at reduced chi, on one host thread and on all the host
+    threads this process may use (at most 16), within about `budget_s`
+    seconds.  `value` is the all-thread rate in the GPU line's unit:
+    algorithmic flops of the four-stage order at the largest sampled chi
+    over the CPU's time per matvec (the same work unit, so the ratio of
+    rates is the ratio of times).  A power law t = a chi^p fitted to each
+    series extrapolates the time per matvec to the chi in `targets`."""
+    import subprocess
+    exe = os.path.join(HERE, "oracle", "_build", "refmatvec")
+    if not os.path.exists(exe):
+        sys.path.insert(0, HERE)
+        from oracle import build as ob
+        exe = str(ob.build_refmatvec())
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(nproc, 16))
+
+    def run(chi, t):
+        r = subprocess.run([exe, "--chi", str(chi), "--d", str(d), "--threads", str(t)],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"refmatvec failed: {r.stderr}")
+        return json.loads(r.stdout)
+
+    t_start = time.perf_counter()
+    series = {}
+    for t, chis in ((1, (8, 16, 24)), (threads, (16, 24, 32, 40))):
+        pts = []
+        for chi in chis:
+            if pts and time.perf_counter() - t_start > budget_s * (0.4 if t == 1 else 1.0):
+                break
+            rec = run(chi, t)
+            pts.append(rec)
+            log(f"cpu baseline: chi={chi} threads={t}: {rec['s_per_matvec']:.3f} s per matvec")
+        series[t] = pts
+
+    def summary(pts):
+        chis = np.array([p["chi"] for p in pts], float)
+        ts = np.array([p["s_per_matvec"] for p in pts], float)
+        out = {"chi": [int(c) for c in chis], "s_per_matvec": [round(float(x), 4) for x in ts],
+               "tflops": [round(matvec_flops_z2(c, d) / x / 1e12, 8) for c, x in zip(chis, ts)],
+               "own_gflops": [round(8.0 * p["complex_macs"] / p["s_per_matvec"] / 1e9, 3) for p in pts]}
+        if len(pts) >= 2:
+            # the two largest samples: the reference's work grows as chi^3 d^3
+            p_exp, loga = np.polyfit(np.log(chis[-2:]), np.log(ts[-2:]), 1)
+            out["fit"] = {"exponent": round(float(p_exp), 3),
+                          "s_per_matvec_at": {str(c): float(f"{np.exp(loga) * c ** p_exp:.4g}") for c in targets}}
+        return out
+
+    one, many = summary(series[1]), summary(series[threads])
+    big = series[threads][-1]
+    value = matvec_flops_z2(big["chi"], d) / big["s_per_matvec"] / 1e12
+    return dict(value=round(value, 8), unit="TFLOP/s", cores=threads, kind="port",
+                sample=(f"oracle/refmatvec (C++ reference-order CSC SpGEMM matvec), d={d} Z2 coupled chain, "
+                        f"chi={big['chi']}, n={big['n']}, {threads} threads: {big['s_per_matvec']:.3f} s per "
+                        f"matvec; flops counted in the four-stage order ({matvec_flops_z2(big['chi'], d):.3e}"
+                        f"/matvec); the reference order itself performs {8 * big['complex_macs']:.3e}"),
+                nproc=nproc, cpu_model=_cpu_model(), single_thread=one, all_threads=many,
+                memory_note=("the reference order holds LeftPart (chi d x chi (d+2)/2 entries), RightPart "
+                             "(the same) and an intermediate of 1.5 chi^2 d^2 entries at 24 bytes each: "
+                             "about 0.8 TB at chi=2000, d=64"),
+                wall_s=round(time.perf_counter() - t_start, 1))
 
 
 def orthogonalise_step(dm, W):
@@ -432,9 +442,12 @@ def main():
         if not a.no_idmrg:
             small["idmrg_step"] = idmrg_steps(ctx, cam, workloads, a)
     batching = None if a.no_batching else batching_section(ctx, cam, workloads, a.steps, a.warmup)
-    cpu = None if a.no_cpu else cpu_baseline(a.cpu_budget)
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_budget, a.d, targets=(a.chi_small, a.chi))
     fp64, fp32 = head["fp64"], head["fp32"]
     names = ["L", "W1", "W2", "R"]
+    if cpu and "fit" in cpu["all_threads"]:
+        t_cpu = cpu["all_threads"]["fit"]["s_per_matvec_at"][str(a.chi)]
+        cpu["fit_time_ratio_cpu_over_gpu_fp64_at_chi"] = {str(a.chi): float(f"{t_cpu / (fp64['ms_per_step'] / 1e3):.4g}")}
     line = {
         "metric": "H_eff matvec TFLOP/s + iDMRG step time, chi=2000 d=64, FP32-mixed vs FP64",
         "value": fp64["tflops"],
@@ -451,7 +464,8 @@ def main():
         "config": {"workload": f"two-vertex H_eff matvec, d={a.d} Z2 coupled chain, w=3, chi={a.chi} "
                                f"(2 parity sectors), vectors resident in HBM",
                    "chi": a.chi, "d": a.d, "mpo_bond": 3, "n": head["n"],
-                   "flops_per_matvec": head["flops_per_matvec"], "parallelism": "single GPU"},
+                   "flops_per_matvec": head["flops_per_matvec"],
+                   "flops_closed_form": matvec_flops_z2(a.chi, a.d), "parallelism": "single GPU"},
         "fp32_split": {"tflops": fp32["tflops"], "ms_per_step": fp32["ms_per_step"],
                        "speedup_vs_fp64": head["speedup_fp32_vs_fp64"],
                        "vectors": "FP32 split complex in the block layout, as the mixed-precision eigensolver "

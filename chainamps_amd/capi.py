@@ -92,6 +92,7 @@ _SIGS = {
     "cam_tebd_gate": [_p, _p, _p, _p, _dp, C.c_int64, C.c_int64, C.c_double, C.POINTER(_p),
                       C.POINTER(_p), _dp, C.c_int64, _i64p, _dp],
     "cam_heff_vjp": [_p, _p, _p, _p, C.c_int64],
+    "cam_heff_coupling_grad": [_p, _p, C.c_int64, C.c_int, _i64p, _i64p, _dp, _dp, _p],
 }
 
 
@@ -437,6 +438,34 @@ class Heff:
         dW = np.zeros(max(nnzW, 1), np.complex128)
         _check(_LIB.cam_heff_vjp(self.h, psi.ctypes.data, dpsi.ctypes.data, dW.ctypes.data, nnzW))
         return dpsi, dW[:nnzW]
+
+    def coupling_grad(self, psi, nnzW, generators, want_dpsi=False):
+        """dE/d(theta_c) for W linear in couplings theta_c: `generators` is a
+        list of (entries, coefs) pairs, dW/dtheta_c over W's stored entries
+        (CSC order) with complex coefficients.  Returns the complex array of
+        dE/dtheta_c (imaginary parts vanish for a Hermitian MPO) and, when
+        asked, dE/dconj(psi)."""
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        ptr = np.zeros(len(generators) + 1, np.int64)
+        ent, cof = [], []
+        for c, (e, v) in enumerate(generators):
+            e = np.asarray(e, np.int64).ravel()
+            v = np.broadcast_to(np.asarray(v, np.complex128), e.shape)
+            ptr[c + 1] = ptr[c] + len(e)
+            ent.append(e)
+            cof.append(v)
+        ent = np.ascontiguousarray(np.concatenate(ent) if ent else np.zeros(1, np.int64))
+        cof = np.ascontiguousarray(np.concatenate(cof) if cof else np.zeros(1, np.complex128))
+        if len(ent) == 0:
+            ent, cof = np.zeros(1, np.int64), np.zeros(1, np.complex128)
+        dE = np.zeros(max(len(generators), 1), np.complex128)
+        dpsi = np.zeros(self.n, np.complex128) if want_dpsi else None
+        _check(_LIB.cam_heff_coupling_grad(self.h, psi.ctypes.data, nnzW, len(generators), _ptr(ptr, C.c_int64),
+                                           _ptr(ent, C.c_int64), _ptr(cof.view(np.float64), C.c_double),
+                                           _ptr(dE.view(np.float64), C.c_double),
+                                           dpsi.ctypes.data if want_dpsi else None))
+        dE = dE[:len(generators)]
+        return (dE, dpsi) if want_dpsi else dE
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,7 +1,9 @@
 // C ABI (include/chainamps_amd.h).  Every entry point converts exceptions
 // into a non-zero status plus cam_last_error() text.
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/chainamps_amd.h"
 #include "ctx.hpp"
@@ -520,6 +522,41 @@ int cam_heff_vjp(cam_heff* h, const double* psi, double* dpsi, double* dW, int64
   set_device(h->owner);
   heff_vjp(h->h, reinterpret_cast<const cplx*>(psi), reinterpret_cast<cplx*>(dpsi),
            reinterpret_cast<cplx*>(dW), nnzW);
+  CAM_API_END
+}
+
+int cam_heff_coupling_grad(cam_heff* h, const double* psi, int64_t nnzW, int ncoup, const int64_t* gen_ptr,
+                           const int64_t* gen_entry, const double* gen_coef, double* dE, double* dpsi) {
+  CAM_API_BEGIN
+  CAM_REQUIRE(ncoup >= 0 && nnzW >= 0, "coupling_grad: negative count");
+  CAM_REQUIRE(ncoup == 0 || (gen_ptr && dE), "coupling_grad: null generator table");
+  CAM_REQUIRE(ncoup == 0 || gen_ptr[0] == 0, "coupling_grad: gen_ptr[0] must be 0");
+  for (int c = 0; c < ncoup; ++c) {
+    CAM_REQUIRE(gen_ptr[c + 1] >= gen_ptr[c], "coupling_grad: gen_ptr must be non-decreasing");
+    for (int64_t k = gen_ptr[c]; k < gen_ptr[c + 1]; ++k)
+      CAM_REQUIRE(gen_entry[k] >= 0 && gen_entry[k] < nnzW, "coupling_grad: generator entry outside W");
+  }
+  set_device(h->owner);
+  std::vector<cplx> own;
+  cplx* dp = reinterpret_cast<cplx*>(dpsi);
+  if (!dp) {
+    own.resize(static_cast<size_t>(h->h.n));
+    dp = own.data();
+  }
+  std::vector<cplx> dW(static_cast<size_t>(std::max<int64_t>(nnzW, 1)));
+  heff_vjp(h->h, reinterpret_cast<const cplx*>(psi), dp, dW.data(), nnzW);
+  // chain rule through W = sum_c theta_c dW/dtheta_c (W linear in the couplings)
+  for (int c = 0; c < ncoup; ++c) {
+    double re = 0.0, im = 0.0;
+    for (int64_t k = gen_ptr[c]; k < gen_ptr[c + 1]; ++k) {
+      const cplx g = dW[gen_entry[k]];
+      const double cr = gen_coef[2 * k], ci = gen_coef[2 * k + 1];
+      re += g.re * cr - g.im * ci;
+      im += g.re * ci + g.im * cr;
+    }
+    dE[2 * c] = re;
+    dE[2 * c + 1] = im;
+  }
   CAM_API_END
 }
 

@@ -512,6 +512,12 @@ void GemmPlan::add_segment(int prob, const void* A, int64_t csA, int64_t bsA, co
   GemmProb& p = probs[prob];
   CAM_REQUIRE(p.seg0 + p.nseg == static_cast<int32_t>(segs.size()),
               "segments of a problem must be added contiguously");
+  // the kernels address an element of a tile by a 32-bit offset from the
+  // tile's first element: at most 128 rows / columns (and KC <= 16 k) times
+  // the stride must stay below 2^31
+  constexpr int64_t kOffLimit = (int64_t(1) << 31) - 256;
+  CAM_REQUIRE(csA >= 0 && ldB >= 0 && csA * 128 < kOffLimit && ldB * 128 < kOffLimit,
+              "add_segment: operand stride too large for 32-bit tile offsets");
   GemmSeg g{};
   g.A = A;
   g.B = B;

@@ -84,6 +84,27 @@ def coupled_chain_mpo(ctx, d=64, seed=0, Jperp=0.1):
     return tensor_from_dense(ctx, rules, idx, 2, T), rules, phys, bq
 
 
+def coupled_chain_generators(csc, d=64, seed=0):
+    """Coupling generators of the coupled-chain MPO of coupled_chain_mpo, in
+    the form Heff.coupling_grad takes: [dW/dJperp, dW/dE_0, ..., dW/dE_{d-1}]
+    as (entries, coefs) over W's stored entries, csc = W.to_csc() (row
+    s' + d*b_left, column s + d*b_right).  W must have been built with
+    Jperp != 0 so that its (1, 0) channel is stored."""
+    rows, cols, p, i, x = csc
+    E, parity, sig = coupled_chain_spectrum(d, seed)
+    col = np.repeat(np.arange(cols), np.diff(p))
+    bl, sl = i // d, i % d
+    br, sr = col // d, col % d
+    jk = np.nonzero((bl == 1) & (br == 0))[0]
+    if len(jk) == 0:
+        raise ValueError("coupled_chain_generators: W has no stored Jperp channel (built with Jperp = 0?)")
+    gens = [(jk, sig[sl[jk], sr[jk]].astype(np.complex128))]
+    for n in range(d):
+        k = np.nonzero((bl == 2) & (br == 0) & (sl == n) & (sr == n))[0]
+        gens.append((k, np.ones(len(k), np.complex128)))
+    return gens
+
+
 def _hermitian_channels(T, mask, rows, cols):
     """Make every MPO channel b of a random environment a Hermitian
     (a' x a) matrix, so that H_eff is Hermitian for a Hermitian MPO whose

@@ -450,6 +450,18 @@ int cam_itwovertex_energy(const cam_tensor* W, const cam_unitcell* u, double* en
  * recomputed, never stored per Lanczos step. */
 int cam_heff_vjp(cam_heff* h, const double* psi, double* dpsi, double* dW, int64_t nnzW);
 
+/* dE/d(MPO couplings) (no counterpart in the reference): the MPO is taken
+ * linear in couplings theta_c, W = sum_c theta_c G_c (+ constant entries),
+ * and each generator G_c = dW/dtheta_c is given as a sparse list over W's
+ * stored entries (cam_tensor_to_csc order): entries
+ * gen_entry[gen_ptr[c] .. gen_ptr[c+1]) with complex coefficients gen_coef
+ * (interleaved re, im).  dE[2c], dE[2c+1] = sum_k dE/dW[entry k] * coef k
+ * (re, im; the imaginary part vanishes for a Hermitian MPO and is returned
+ * as a check).  dpsi (host, n complex, reference order) receives dE/dconj(psi)
+ * when not NULL.  Same cost as cam_heff_vjp. */
+int cam_heff_coupling_grad(cam_heff* h, const double* psi, int64_t nnzW, int ncoup, const int64_t* gen_ptr,
+                           const int64_t* gen_entry, const double* gen_coef, double* dE, double* dpsi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,14 @@ def test_hellmann_feynman_couplings(ctx):
         return cam.Heff(to_lib(ctx, LB), to_lib(ctx, Wj), to_lib(ctx, RB)).lowest(converge=True)[0]
     fd = (lowest(J + hh) - lowest(J - hh)) / (2 * hh)
     assert abs(dEdJ - fd) <= 1e-6 * max(1.0, abs(fd)), (dEdJ, fd)
+    # the library's chain rule over the coupling generators
+    from chainamps_amd import workloads
+    csc = (m.shape[0], m.shape[1], m.indptr.astype(np.int64), m.indices.astype(np.int64), m.data)
+    g, dpsi2 = H.coupling_grad(psi, m.nnz, workloads.coupled_chain_generators(csc, d, seed=2), want_dpsi=True)
+    assert len(g) == d + 1
+    assert abs(g[0] - dEdJ) <= 1e-12 * max(1.0, abs(dEdJ)), (g[0], dEdJ)
+    assert abs(g[0].real - fd) <= 1e-6 * max(1.0, abs(fd))
+    assert np.max(np.abs(dpsi2 - dpsi)) <= 1e-11      # same vjp, run again
     # dE/dE_n for every level: chain rule through W[2,0] = diag(E), against
     # the (exact) central difference of E(psi; E_n)
     for nlev in range(d):
@@ -107,3 +115,4 @@ def test_hellmann_feynman_couplings(ctx):
         Em[nlev] -= h
         ref = (_energy(LB, _model_W(d, J, Ep)[0], RB, psi) - _energy(LB, _model_W(d, J, Em)[0], RB, psi)) / (2 * h)
         assert abs(dW[k[0]].real - ref) <= 1e-9 * max(1.0, abs(ref)), (nlev, dW[k[0]], ref)
+        assert abs(g[1 + nlev] - dW[k[0]]) <= 1e-12 * max(1.0, abs(dW[k[0]]))

@@ -208,6 +208,14 @@ def test_fullsize_adjoint(idmrg_heff):
         k = np.where((bl == 2) & (br == 0) & (i % D == nlev) & (col_of % D == nlev))[0]
         assert len(k) == 1
         lev[nlev] = dW[k[0]].real
+    # the library's chain rule (cam_heff_coupling_grad) against the assembly above
+    from chainamps_amd import workloads
+    gens = workloads.coupled_chain_generators((rows, cols, p, i, x), D, seed=0)
+    g = H.coupling_grad(psi0, nnz, gens)
+    print(f"  coupling_grad dE/dJ {g[0].real:.12f}, max |Im| {np.max(np.abs(g.imag)):.2e}", flush=True)
+    assert abs(g[0] - dEdJ) <= 1e-12 * max(1.0, abs(dEdJ))
+    assert np.max(np.abs(g[1:] - lev)) <= 1e-12 * max(1.0, np.max(np.abs(lev)))
+    assert np.max(np.abs(g.imag)) <= 1e-10 * max(1.0, np.max(np.abs(g)))
     E = idmrg_heff["E"]
     rng = np.random.default_rng(11)
     for trial in range(2):

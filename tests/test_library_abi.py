@@ -63,3 +63,38 @@ def test_synthetic_generator_matches_oracle_models():
     qs, sz = workloads.sector_profile(1024)
     qs2, sz2 = models.sector_profile(1024)
     assert np.array_equal(sz, sz2) and sz.sum() == 1024 and len(qs) == 21
+
+
+def test_coupled_chain_generators():
+    """The coupling generators handed to cam_heff_coupling_grad are
+    dW/dJperp and dW/dE_n of the coupled-chain MPO: W(J, E) - W(0, E) = J G_J
+    and W(J, E + u) - W(J, E) = sum_n u_n G_{E_n} entry by entry."""
+    from chainamps_amd import workloads
+    from oracle import models
+    d, seed, J = 8, 5, 0.37
+    rules, phys, W, E, sig = models.coupled_chain_model(d=d, seed=seed, Jperp=J)
+    m = W.matrix.tocsc()
+    m.sort_indices()
+    csc = (m.shape[0], m.shape[1], m.indptr.astype(np.int64), m.indices.astype(np.int64), m.data)
+    gens = workloads.coupled_chain_generators(csc, d, seed=seed)
+    assert len(gens) == d + 1
+
+    def dense(gen):
+        out = np.zeros(m.nnz, complex)
+        np.add.at(out, gen[0], gen[1])
+        return out
+    # W linear in J: the J generator times J is the whole (1, 0) channel
+    W0 = models.coupled_chain_model(d=d, seed=seed, Jperp=1e-300)[2].matrix.tocsc()
+    W0.sort_indices()
+    assert np.allclose(J * dense(gens[0]), m.data - W0.data, rtol=0, atol=1e-15)
+    u = np.random.default_rng(1).standard_normal(d)
+    Wu = models.coupled_chain_model(d=d, seed=seed, Jperp=J, E=E + u)[2].matrix.tocsc()
+    Wu.sort_indices()
+    step = sum(u[n] * dense(gens[1 + n]) for n in range(d))
+    assert np.allclose(step, Wu.data - m.data, rtol=0, atol=1e-14)
+    with pytest.raises(ValueError):
+        W00 = models.coupled_chain_model(d=d, seed=seed, Jperp=0.0)[2].matrix.tocsc()
+        W00.sort_indices()
+        workloads.coupled_chain_generators(
+            (W00.shape[0], W00.shape[1], W00.indptr.astype(np.int64), W00.indices.astype(np.int64), W00.data),
+            d, seed=seed)
