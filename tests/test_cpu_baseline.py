@@ -37,6 +37,10 @@ def test_refmatvec_against_dense(refmatvec, chi, d, threads, seed):
     assert rec["check_rel_err"] <= 1e-13, rec
     # the reference order's work: LeftPart has chi/2 (d + 2) entries per column
     assert rec["nz_left_part"] == chi * d * (chi // 2) * (d + 2)
+    # the multiply-add count bench.py extrapolates with
+    sys.path.insert(0, ROOT)
+    import bench
+    assert rec["complex_macs"] == bench.ref_macs(chi, d)
 
 
 def test_refmatvec_rejects_bad_arguments(refmatvec):
