@@ -25,12 +25,18 @@ struct Ctx {
   // Freeing and re-allocating ~10^11-byte buffers between eigensolves costs
   // seconds near the memory limit, so they are kept until the context ends.
   DevBuf<cplx> ws_krylov, ws_t1, ws_t2;
-  // Leading-values SVD: per block shape (m, n, k), the subspace rounds
+  // Leading-values SVD: per block shape (m, n, k) and number of checked
+  // triplets, the subspace rounds
   // before the first residual check (the rounds the last accepted block of
   // that shape needed; lowered by one after three blocks in a row were
   // accepted at their first check) and that streak.  It changes only how
   // many rounds run, never the acceptance test.
-  std::map<std::array<int64_t, 3>, std::array<int, 2>> svd_round_hint;
+  std::map<std::array<int64_t, 4>, std::array<int, 2>> svd_round_hint;
+  // Leading-values SVD: per block shape (m, n, k) and block charge, how many
+  // of its values the last truncation kept (the triplets checked first in
+  // the next block of that shape; more are checked when the truncation
+  // keeps more)
+  std::map<std::pair<std::array<int64_t, 3>, Charge>, int64_t> svd_kept_hint;
   explicit Ctx(int dev);
   ~Ctx();
   void* blas_handle();

@@ -127,9 +127,36 @@ bool orthonormal(Ctx& ctx, const cplx* X, int64_t m, int64_t k, cplx* G) {
   return v <= 1e-12;
 }
 
+// ratio[0] = min_i |R_ii| / max_i |R_ii| of a k x k Cholesky factor (one
+// workgroup); zero when the factorization reported a breakdown
+__global__ void pivot_ratio_kernel(const double2* __restrict__ R, int64_t k, const int* __restrict__ info,
+                                   double* __restrict__ ratio) {
+  __shared__ double lo[256], hi[256];
+  double a = INFINITY, b = 0.0;
+  for (int64_t i = threadIdx.x; i < k; i += blockDim.x) {
+    const double v = fabs(R[i + i * k].x);
+    a = fmin(a, v);
+    b = fmax(b, v);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + o]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ratio[0] = (info[0] != 0 || !(hi[0] > 0.0)) ? 0.0 : lo[0] / hi[0];
+}
+
 // One Cholesky QR pass: G = X^H X (+ shift), R = chol(G), X <- X R^{-1}.
-// False when the factorization breaks down.
-bool cholqr_pass(Ctx& ctx, cplx* X, int64_t m, int64_t k, cplx* G, int* dinfo, double shift) {
+// False when the factorization breaks down or, with min_pivot > 0, when
+// min |R_ii| < min_pivot max |R_ii| (X too close to rank deficient for
+// one pass); X is then unchanged.
+bool cholqr_pass(Ctx& ctx, cplx* X, int64_t m, int64_t k, cplx* G, int* dinfo, double shift,
+                 double min_pivot = 0.0, double* dratio = nullptr) {
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   hipStream_t s = ctx.stream;
   // the Gram matrix with zgemm: rocBLAS zherk takes 13 ms where zgemm takes
@@ -142,10 +169,20 @@ bool cholqr_pass(Ctx& ctx, cplx* X, int64_t m, int64_t k, cplx* G, int* dinfo, d
   }
   st = rocsolver_zpotrf(h, rocblas_fill_upper, static_cast<rocblas_int>(k), Z(G), static_cast<rocblas_int>(k), dinfo);
   CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zpotrf failed");
-  int info = 0;
-  HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  if (info != 0) return false;
+  if (min_pivot > 0.0) {
+    hipLaunchKernelGGL(pivot_ratio_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<const double2*>(G), k, dinfo,
+                       dratio);
+    HIP_CHECK(hipGetLastError());
+    double ratio = 0.0;
+    HIP_CHECK(hipMemcpyAsync(&ratio, dratio, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (!(ratio >= min_pivot)) return false;
+  } else {
+    int info = 0;
+    HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (info != 0) return false;
+  }
   st = rocblas_ztrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
                      static_cast<rocblas_int>(m), static_cast<rocblas_int>(k), &kOne, Z(G),
                      static_cast<rocblas_int>(k), Z(X), static_cast<rocblas_int>(m));
@@ -183,12 +220,13 @@ void orthonormalise(Ctx& ctx, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& G, De
 
 bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int64_t ncheck, double tol,
                    int max_rounds, std::vector<double>& S, DevBuf<cplx>& U, DevBuf<cplx>& Vh, int* rounds,
-                   double* last_res) {
+                   double* last_res, int64_t naccept, std::vector<double>* res_out) {
   CAM_REQUIRE(k >= 1 && k <= std::min(m, n), "truncated_svd: k out of range");
   CAM_REQUIRE(m < (int64_t(1) << 31) && n < (int64_t(1) << 31), "truncated_svd: block too large");
   hipStream_t s = ctx.stream;
   rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
   ncheck = std::min(ncheck, k);
+  naccept = (naccept <= 0) ? ncheck : std::min(naccept, ncheck);
   DevBuf<cplx> Y(m * k), W(n * k), Wc(n * k), R(k * k), Ur(k * k), Vrh(k * k), E(m * std::max<int64_t>(ncheck, 1)),
       tau, G;
   DevBuf<int> dinfo;
@@ -201,7 +239,10 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   // accepted block of this shape on this context needed, since the blocks
   // of consecutive iDMRG / iTEBD steps converge alike (the hint only moves
   // the check; acceptance is always the residual test below).
-  const std::array<int64_t, 3> shape{m, n, k};
+  // (the number of checked triplets sets the pace: the slowest of them
+  // converges at the rate (s_{k+1} / s_ncheck)^2 per round, so the hints
+  // are kept per ncheck as well)
+  const std::array<int64_t, 4> shape{m, n, k, naccept};
   const int first_default = (m * n > (int64_t(1) << 24)) ? 4 : 2;
   int first = first_default;
   {
@@ -221,15 +262,27 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
   };
   double t_round = 0, t_proj = 0, t_core = 0, t_res = 0;
   auto tp = std::chrono::steady_clock::now();
+  // Between checks only the span of the iterates matters: one Cholesky QR
+  // pass per multiplication keeps the condition of the next iterate near
+  // that of A (its orthogonality error, about u cond^2, is irrelevant to
+  // the span); where that pass breaks down the robust orthonormalisation
+  // takes over.  The Rayleigh-Ritz step below always orthonormalises fully.
+  G.ensure(static_cast<size_t>(k) * k);
+  dinfo.ensure(1);
+  DevBuf<double> dratio(1);
+  auto span_step = [&](cplx* X, int64_t rows) {
+    if (!cholqr_pass(ctx, X, rows, k, G.get(), dinfo.get(), 0.0, 1e-7, dratio.get()))
+      orthonormalise(ctx, X, rows, k, G, dinfo, tau);
+  };
   for (;;) {
     // before the first residual check: four subspace rounds on blocks of
     // more than 2^24 elements (they have needed at least six), else two, or
     // the rounds the last block of this shape needed; then two between checks
     const int nr = q == 0 ? first : 2;
     for (int r = 0; r < nr; ++r, ++q) {
-      orthonormalise(ctx, Y.get(), m, k, G, dinfo, tau);
+      span_step(Y.get(), m);
       gemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, k, m, A, m, Y.get(), m, W.get(), n);
-      orthonormalise(ctx, W.get(), n, k, G, dinfo, tau);
+      span_step(W.get(), n);
       gemm(h, rocblas_operation_none, rocblas_operation_none, m, k, n, A, m, W.get(), n, Y.get(), m);
     }
     tick(t_round, tp);
@@ -268,15 +321,19 @@ bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int
     HIP_CHECK(hipMemcpyAsync(res.data(), dres.get(), ncheck * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     double worst = 0.0;
-    for (double r : res) worst = std::max(worst, std::sqrt(r));
+    for (int64_t j = 0; j < naccept; ++j) worst = std::max(worst, std::sqrt(res[j]));
     tick(t_res, tp);
     if (trace)
-      std::fprintf(stderr, "truncated_svd %lld x %lld k=%lld: after %d rounds worst residual %.3e (tol %.3e); "
-                   "rounds %.3f s, projection %.3f s, core %.3f s, residual %.3f s\n",
-                   static_cast<long long>(m), static_cast<long long>(n), static_cast<long long>(k), q, worst, tol,
-                   t_round, t_proj, t_core, t_res);
+      std::fprintf(stderr, "truncated_svd %lld x %lld k=%lld: after %d rounds worst residual of the first %lld "
+                   "%.3e (tol %.3e); rounds %.3f s, projection %.3f s, core %.3f s, residual %.3f s\n",
+                   static_cast<long long>(m), static_cast<long long>(n), static_cast<long long>(k), q,
+                   static_cast<long long>(naccept), worst, tol, t_round, t_proj, t_core, t_res);
     if (rounds) *rounds = q;
     if (worst <= tol) {
+      if (res_out) {
+        res_out->resize(ncheck);
+        for (int64_t j = 0; j < ncheck; ++j) (*res_out)[j] = std::sqrt(res[j]);
+      }
       if (last_res) {
         // residual of the last computed triplet (the slowest to converge):
         // bounds how far its value may sit below a singular value of A

@@ -34,9 +34,12 @@ void scale_along_index(Ctx& ctx, BSTensor& T, int k, const std::vector<double>& 
 // Vh is k x n, S descending.  last_res (if given) receives the residual
 // |A v_k - s_k u_k| of the last computed triplet at acceptance (0 when
 // ncheck == k, whose residuals are all within tol).
+// With naccept (0 < naccept <= ncheck) only the first naccept residuals
+// must be <= tol; res (if given) receives all ncheck residuals of the
+// accepted result.
 bool truncated_svd(Ctx& ctx, const cplx* A, int64_t m, int64_t n, int64_t k, int64_t ncheck, double tol,
                    int max_rounds, std::vector<double>& S, DevBuf<cplx>& U, DevBuf<cplx>& Vh, int* rounds,
-                   double* last_res = nullptr);
+                   double* last_res = nullptr, int64_t naccept = -1, std::vector<double>* res = nullptr);
 
 // Hermitian eigendecomposition of the n x n device matrix A (column-major):
 // on return A holds the eigenvectors, w (device, n) the eigenvalues in

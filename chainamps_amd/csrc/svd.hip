@@ -132,6 +132,9 @@ struct Group {
   bool truncated = false;                        // only the leading k values computed
   double tail = 0.0;                             // weight - sum S^2 when truncated
   double last_res = 0.0;                         // residual of the last computed triplet
+  int64_t kk = 0;                                // leading triplets asked for (truncated path)
+  int64_t nver = 0;                              // leading triplets verified at acceptance
+  std::vector<double> res;                       // residuals of the first ncheck triplets
 };
 
 }  // namespace
@@ -330,16 +333,30 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
       std::vector<double> S;
       int rounds = 0;
       const auto t0 = std::chrono::steady_clock::now();
-      const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, std::min(D, kk), kRitzTol * std::sqrt(g.weight),
-                                    24, S, g.U, g.Vh, &rounds, &g.last_res);
+      // residuals of the first min(D, kk) triplets are computed; the ones
+      // that must be within the tolerance for acceptance are the first
+      // (kept count of the last block of this shape and charge) + margin,
+      // or all of them without that hint.  The truncation below verifies
+      // that everything it keeps was within the tolerance.
+      const int64_t ncheck = std::min(D, kk);
+      int64_t naccept = ncheck;
+      const auto hk = std::make_pair(std::array<int64_t, 3>{g.mg, g.ng, kk}, g.q);
+      const auto hit = ctx.svd_kept_hint.find(hk);
+      if (hit != ctx.svd_kept_hint.end())
+        naccept = std::min(ncheck, std::max<int64_t>(32, hit->second + std::max<int64_t>(16, hit->second / 8)));
+      const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, ncheck, kRitzTol * std::sqrt(g.weight), 24, S,
+                                    g.U, g.Vh, &rounds, &g.last_res, naccept, &g.res);
       if (trace)
-        std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld: %s after %d rounds, %.3f s\n",
+        std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld (first %lld verified): %s after %d "
+                     "rounds, %.3f s\n",
                      static_cast<long long>(g.mg), static_cast<long long>(g.ng), static_cast<long long>(kk),
-                     ok ? "accepted" : "not accepted", rounds,
+                     static_cast<long long>(naccept), ok ? "accepted" : "not accepted", rounds,
                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
       if (ok) {
         g.S = std::move(S);
         g.k = kk;
+        g.kk = kk;
+        g.nver = naccept;
         g.truncated = true;
         double kept = 0.0;
         for (double v : g.S) kept += v * v;
@@ -397,12 +414,55 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
     for (size_t i = length; i < nall; ++i) discarded += all[i].v * all[i].v;
     bool redo = false;
     const double last = all[length - 1].v;
-    for (Group* g : glist)
-      if (g->truncated && !(g->S.back() + g->last_res < last * (1.0 - 2.0e-3))) {
-        full_svd(*g);   // each pass turns at least one group exact: at most #groups passes
-        redo = true;
+    const double below = last * (1.0 - 2.0e-3);
+    std::vector<int64_t> kept_of(glist.size(), 0);
+    for (size_t i = 0; i < length; ++i) kept_of[all[i].g]++;
+    for (size_t gi = 0; gi < glist.size(); ++gi) {
+      Group* g = glist[gi];
+      if (!g->truncated) continue;
+      bool safe = g->S.back() + g->last_res < below;
+      // a group accepted on a prefix of its triplets: every value it keeps
+      // must be among the verified ones, and each checked value beyond the
+      // verified prefix, raised by its residual, must lie below the cut
+      const int64_t nres = static_cast<int64_t>(g->res.size());
+      const double tol = kRitzTol * std::sqrt(g->weight);
+      if (safe && g->nver < nres) {
+        if (kept_of[gi] > g->nver) safe = false;
+        for (int64_t j = g->nver; safe && j < nres; ++j)
+          if (j < kept_of[gi] ? g->res[j] > tol : !(g->S[j] + g->res[j] < below)) safe = false;
       }
+      if (safe) continue;
+      redo = true;
+      if (g->nver < nres) {
+        // verify every checked triplet, as without a kept-count hint
+        std::vector<double> S;
+        int rounds = 0;
+        if (truncated_svd(ctx, g->A.get(), g->mg, g->ng, g->kk, nres, tol, 24, S, g->U, g->Vh, &rounds,
+                          &g->last_res, nres, &g->res)) {
+          g->S = std::move(S);
+          g->nver = nres;
+          double kept = 0.0;
+          for (double v : g->S) kept += v * v;
+          g->tail = std::max(0.0, g->weight - kept);
+          if (trace)
+            std::fprintf(stderr, "svd_truncate: block %lld x %lld re-verified on all %lld checked triplets "
+                         "(keeps %lld)\n", static_cast<long long>(g->mg), static_cast<long long>(g->ng),
+                         static_cast<long long>(nres), static_cast<long long>(kept_of[gi]));
+          continue;
+        }
+      }
+      full_svd(*g);   // each pass turns at least one group exact: at most 2 #groups passes
+    }
     if (!redo) break;
+  }
+  // the kept counts steer the next block of the same shape and charge
+  {
+    std::vector<int64_t> kept_of(glist.size(), 0);
+    for (size_t i = 0; i < length; ++i) kept_of[all[i].g]++;
+    for (size_t gi = 0; gi < glist.size(); ++gi)
+      if (glist[gi]->kk > 0)
+        ctx.svd_kept_hint[std::make_pair(std::array<int64_t, 3>{glist[gi]->mg, glist[gi]->ng, glist[gi]->kk},
+                                         glist[gi]->q)] = kept_of[gi];
   }
   for (Group* g : glist) g->A.reset();
   values.clear();
