@@ -771,12 +771,14 @@ void launch_grouped_gemm(GemmPlan& plan, int precision, hipStream_t s, const Gem
     launch_shape<double, 64, 32>(plan, c, 2, s, grid_limit, b);
     launch_shape<double, 128, 64>(plan, c, 3, s, grid_limit, b);
     launch_shape<double, 64, 128>(plan, c, 4, s, grid_limit, b);
+    launch_shape<double, 128, 32>(plan, c, 5, s, grid_limit, b);
   } else {
     launch_shape<float, 64, 64>(plan, c, 0, s, grid_limit, b);
     launch_shape<float, 32, 64>(plan, c, 1, s, grid_limit, b);
     launch_shape<float, 64, 32>(plan, c, 2, s, grid_limit, b);
     launch_shape<float, 128, 64>(plan, c, 3, s, grid_limit, b);
     launch_shape<float, 64, 128>(plan, c, 4, s, grid_limit, b);
+    launch_shape<float, 128, 32>(plan, c, 5, s, grid_limit, b);
   }
 }
 

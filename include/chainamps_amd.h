@@ -187,7 +187,7 @@ int cam_heff_stage_bytes(const cam_heff* h, double* stage_bytes);
  * histogram; CAM_AUTOTUNE=0 disables).  set fixes it for every stage:
  * algo 0 = four real MFMAs per complex product, 1 = three (Gauss);
  * tile_mode 0 = 64x64 tiles, 1 = per-block least padding, 2 = 32x64/64x32
- * only, 3 = least padding including 128x64 / 64x128 tiles; wg_per_cu =
+ * only, 3 = least padding including 128x64 / 64x128 / 128x32 tiles; wg_per_cu =
  * workgroups per CU in the persistent grid; kchunk = k per LDS stage (16,
  * or 8 for half the LDS and more resident workgroups).
  * get returns {algo, tile_mode, wg_per_cu, kchunk} per stage (16 ints). */

@@ -74,3 +74,47 @@ def test_large_block_truncation(ctx, kind, D, trunc, exact):
     assert np.allclose(Vm @ Vm.conj().T, np.eye(length), atol=1e-12)
     proj = Um.conj().T @ A @ Vm.conj().T
     assert np.allclose(proj, np.diag(vals), atol=1e-12)
+
+
+def test_kept_count_hint_reverification(ctx, capfd):
+    """A block whose leading-values decomposition was accepted on the first
+    (last kept count + margin) triplets, but of which the truncation keeps
+    more: the kept triplets beyond the verified prefix must be re-verified
+    (or the block decomposed in full), so the result stays the exact one.
+    Three decompositions of one shape on one context: the split of the kept
+    values between the two blocks swaps in the second."""
+    import chainamps_amd as cam
+    print("case svd kept-count hint re-verification", flush=True)
+    m, D = 2048, 150
+    i = np.arange(m)
+    splits = [(0.45 * 0.98 ** i, 0.98 ** i), (0.98 ** i, 0.45 * 0.98 ** i), (0.98 ** i, 0.45 * 0.98 ** i)]
+    old = os.environ.get("CAM_SVD_TRACE")
+    os.environ["CAM_SVD_TRACE"] = "1"
+    try:
+        for call, spectra in enumerate(splits):
+            norm = np.sqrt(sum(np.sum(s ** 2) for s in spectra))
+            spectra = [s / norm for s in spectra]
+            T, A = _theta(ctx, spectra, m, seed=7 + call)
+            allv = np.sort(np.concatenate(spectra))[::-1]
+            length, kept, disc = O.truncation_rule(list(allv), D, 0.0)
+            U, vals, Vd, d = cam.svd_truncate(T, D, 0.0)
+            err = capfd.readouterr().err
+            kept0 = int(np.sum(spectra[0] >= allv[length - 1]))
+            print(f"  call {call}: block 0 keeps {kept0} of {length}; "
+                  f"re-verified: {'re-verified' in err}", flush=True)
+            if call == 1:
+                # the hint from call 0 (block 0 kept few) is too small now
+                assert "re-verified" in err or "full decomposition" in err
+            assert len(vals) == length
+            assert np.max(np.abs(vals - allv[:length])) <= 1e-13
+            assert abs(d - disc) <= 1e-12
+            rules = ChargeRules([2])
+            Um = from_lib(U, rules).matrix.toarray()
+            Vm = from_lib(Vd, rules).matrix.toarray()
+            proj = Um.conj().T @ A @ Vm.conj().T
+            assert np.allclose(proj, np.diag(vals), atol=1e-12)
+    finally:
+        if old is None:
+            del os.environ["CAM_SVD_TRACE"]
+        else:
+            os.environ["CAM_SVD_TRACE"] = old
