@@ -447,6 +447,142 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmProb* __res
   }
 }
 
+// Skinny problems (N <= kSkinnyN, short segments, A M-contiguous): the W
+// stages of small QN sectors, where a 16x16 MFMA fragment would be mostly
+// padding.  One thread per output row accumulates its N outputs in
+// registers.  Per work item the K x N blocks of B of all the problem's
+// segments are staged in LDS once (every thread reads the same element: a
+// broadcast), and the segments' k steps are flattened into one table of
+// (A column address, B row, operand flags), so the loop keeps eight
+// coalesced A loads in flight across segment boundaries with no barrier.
+// A is read once per problem and C written once: bound by HBM like the
+// copy it nearly is.
+template <typename R>
+__global__ __launch_bounds__(256) void skinny_gemm_kernel(const GemmProb* __restrict__ probs,
+                                                          const GemmSeg* __restrict__ segs,
+                                                          const GemmTile* __restrict__ items, int nitems,
+                                                          GemmBases bases) {
+  using C = typename CT<R>::T;
+  __shared__ R sBr[kSkinnyB], sBi[kSkinnyB];
+  __shared__ const C* sStepA[kSkinnySteps];    // A column of the step, batch and k applied
+  __shared__ int sStepB[kSkinnySteps];         // first element of its B row in sBr / sBi
+  __shared__ int sStepF[kSkinnySteps];         // segment flags
+  __shared__ R sStepAr[kSkinnySteps], sStepAi[kSkinnySteps];
+  __shared__ int sSegK[kSkinnySegs], sSegOff[kSkinnySegs], sSegStep[kSkinnySegs];
+  const int tid = threadIdx.x;
+  // XCD-aware order, as for the tiles: consecutive items (the problems of
+  // one row group over the same rows, which read the same A rows) go to
+  // workgroups of one XCD, whose L2 then serves the repeated reads
+  const int Gd = gridDim.x;
+  const int v0 = (Gd % kXcds == 0) ? (blockIdx.x % kXcds) * (Gd / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+  for (int it = v0; it < nitems; it += Gd) {
+    const GemmTile w = cload(items + it);
+    const GemmProb P = cload(probs + w.prob);
+    const int M = P.M, N = P.N, ns = P.nseg;
+    __syncthreads();   // the previous item's tables are no longer read
+    if (tid < ns) sSegK[tid] = cload(segs + P.seg0 + tid).K;
+    __syncthreads();
+    if (tid == 0) {
+      int off = 0, st = 0;
+      for (int j = 0; j < ns; ++j) {
+        sSegOff[j] = off;
+        sSegStep[j] = st;
+        off += sSegK[j] * N;
+        st += sSegK[j];
+      }
+    }
+    __syncthreads();
+    int nsteps = 0;
+    for (int j = 0; j < ns; ++j) nsteps += sSegK[j];
+    // B blocks and the step table of every segment
+    for (int j = 0; j < ns; ++j) {
+      const GemmSeg G = cload(segs + P.seg0 + j);
+      const int asp = (G.flags >> 8) & 15, bsp = (G.flags >> 12) & 15;
+      const C* Ab = (asp ? reinterpret_cast<const C*>(bases.p[asp]) + reinterpret_cast<intptr_t>(G.A)
+                         : reinterpret_cast<const C*>(G.A)) +
+                    static_cast<int64_t>(w.batch) * G.bsA;
+      const C* Bb = (bsp ? reinterpret_cast<const C*>(bases.p[bsp]) + reinterpret_cast<intptr_t>(G.B)
+                         : reinterpret_cast<const C*>(G.B)) +
+                    static_cast<int64_t>(w.batch) * G.bsB;
+      const bool cB = G.flags & kSegConjB, bnc = G.flags & kSegBNContig;
+      const int K = G.K, off = sSegOff[j], st = sSegStep[j];
+      for (int e = tid; e < K * N; e += 256) {
+        const int k = e / N, n = e % N;
+        const C b = gload(Bb + (bnc ? static_cast<int64_t>(k) * G.ldB + n : k + static_cast<int64_t>(n) * G.ldB));
+        sBr[off + e] = b.x;
+        sBi[off + e] = cB ? -b.y : b.y;
+      }
+      if (tid < K) {
+        sStepA[st + tid] = Ab + static_cast<int64_t>(tid) * G.csA;
+        sStepB[st + tid] = off + tid * N;
+        sStepF[st + tid] = G.flags;
+        sStepAr[st + tid] = static_cast<R>(G.alpha_re);
+        sStepAi[st + tid] = static_cast<R>(G.alpha_im);
+      }
+    }
+    __syncthreads();
+    const int m = w.m0 + tid;
+    if (m >= M) continue;
+    R accr[kSkinnyN], acci[kSkinnyN];
+#pragma unroll
+    for (int n = 0; n < kSkinnyN; ++n) accr[n] = acci[n] = R(0);
+    for (int j0 = 0; j0 < nsteps; j0 += 8) {
+      C a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = gload(sStepA[j0 + u < nsteps ? j0 + u : j0] + m);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (j0 + u >= nsteps) break;
+        const int j = j0 + u, fl = sStepF[j], bo = sStepB[j];
+        R xr = a[u].x, xi = (fl & kSegConjA) ? -a[u].y : a[u].y;
+        if (fl & kSegAlpha) {
+          const R are = sStepAr[j], aim = sStepAi[j];
+          const R tr = xr * are - xi * aim;
+          xi = xr * aim + xi * are;
+          xr = tr;
+        }
+#pragma unroll
+        for (int n = 0; n < kSkinnyN; ++n) {
+          if (n >= N) break;
+          const R br = sBr[bo + n], bi = sBi[bo + n];
+          accr[n] += xr * br - xi * bi;
+          acci[n] += xr * bi + xi * br;
+        }
+      }
+    }
+    C* Cp = (P.cspace ? reinterpret_cast<C*>(const_cast<void*>(bases.p[P.cspace])) + reinterpret_cast<intptr_t>(P.C)
+                      : reinterpret_cast<C*>(P.C)) +
+            static_cast<int64_t>(w.batch) * P.bsC + m;
+#pragma unroll
+    for (int n = 0; n < kSkinnyN; ++n) {
+      if (n >= N) break;
+      C* dst = Cp + static_cast<int64_t>(n) * P.csC;
+      C val;
+      val.x = accr[n];
+      val.y = acci[n];
+      if (P.beta) {
+        const C old = gload(dst);
+        val.x += old.x;
+        val.y += old.y;
+      }
+      gstore(dst, val);
+    }
+  }
+}
+
+template <typename R>
+void launch_skinny(const GemmPlan& plan, const LaunchCfg& c, hipStream_t s, int grid_limit, const GemmBases& bases) {
+  const int n = static_cast<int>(plan.skinny.size());
+  if (!n) return;
+  static int cus = device_cu_count();
+  int grid = std::min(n, cus * 8);
+  if (grid_limit > 0) grid = std::min(grid, grid_limit);
+  (void)c;
+  hipLaunchKernelGGL((skinny_gemm_kernel<R>), dim3(grid), dim3(256), 0, s, plan.d_probs.get(), plan.d_segs.get(),
+                     plan.d_skinny.get(), n, bases);
+  HIP_CHECK(hipGetLastError());
+}
+
 template <typename R, int TM, int TN>
 void launch_shape(const GemmPlan& plan, const LaunchCfg& c, int shape, hipStream_t s, int grid_limit,
                   const GemmBases& bases) {
@@ -546,12 +682,36 @@ void GemmPlan::build_tiles() {
   if (cfg[1].tile_mode != cfg[0].tile_mode) build_tiles_mode(cfg[1].tile_mode);
 }
 
+static bool skinny_eligible(const GemmPlan& plan, const GemmProb& p) {
+  static const bool off = [] {
+    const char* e = std::getenv("CAM_SKINNY");
+    return e && *e == '0';
+  }();
+  if (off || p.N > kSkinnyN || p.nseg == 0 || p.nseg > kSkinnySegs) return false;
+  int64_t steps = 0;
+  for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
+    const GemmSeg& g = plan.segs[s];
+    if (g.K > kSkinnyK || (g.flags & kSegAKContig)) return false;
+    steps += g.K;
+  }
+  // the step table and every B block fit the kernel's LDS tables
+  return steps <= kSkinnySteps && steps * p.N <= kSkinnyB;
+}
+
 void GemmPlan::build_tiles_mode(int mode) {
   auto& T = tiles[mode];
   for (int t = 0; t < kNumTileShapes; ++t) T[t].clear();
   std::vector<double> cost[kNumTileShapes];
+  skinny.clear();
+  skinny_uploaded = false;
   for (size_t pi = 0; pi < probs.size(); ++pi) {
     const GemmProb& p = probs[pi];
+    if (skinny_eligible(*this, p)) {
+      for (int b = 0; b < p.batch; ++b)
+        for (int m0 = 0; m0 < p.M; m0 += kSkinnyRows)
+          skinny.push_back(GemmTile{static_cast<int32_t>(pi), b, m0, 0});
+      continue;
+    }
     int shape = 0;
     if (mode != 0) {
       double best = 1e300;
@@ -622,6 +782,16 @@ void GemmPlan::build_tiles_mode(int mode) {
     for (auto o : ord) sorted.push_back(T[t][o]);
     T[t].swap(sorted);
   }
+  // skinny items: the problems of one row group (same A rows) next to each
+  // other at equal (batch, m0), so that they run together on one XCD
+  std::stable_sort(skinny.begin(), skinny.end(), [&](const GemmTile& x, const GemmTile& y) {
+    const int64_t gx = group[x.prob] >= 0 ? group[x.prob] : -1 - x.prob;
+    const int64_t gy = group[y.prob] >= 0 ? group[y.prob] : -1 - y.prob;
+    if (gx != gy) return gx < gy;
+    if (x.batch != y.batch) return x.batch < y.batch;
+    if (x.m0 != y.m0) return x.m0 < y.m0;
+    return x.prob < y.prob;
+  });
   built[mode] = true;
   tiles_uploaded[mode] = false;
 }
@@ -633,6 +803,11 @@ void GemmPlan::upload(hipStream_t s) {
     d_segs.ensure(std::max<size_t>(segs.size(), 1));
     d_segs.upload(segs.data(), segs.size(), s);
     uploaded = true;
+  }
+  if (!skinny_uploaded) {
+    d_skinny.ensure(std::max<size_t>(skinny.size(), 1));
+    d_skinny.upload(skinny.data(), skinny.size(), s);
+    skinny_uploaded = true;
   }
   for (int m = 0; m < kTileModes; ++m) {
     if (!built[m] || tiles_uploaded[m]) continue;
@@ -648,6 +823,8 @@ void GemmPlan::clear() {
   probs.clear();
   segs.clear();
   group.clear();
+  skinny.clear();
+  skinny_uploaded = false;
   for (int m = 0; m < kTileModes; ++m) {
     for (auto& t : tiles[m]) t.clear();
     built[m] = false;
@@ -772,6 +949,7 @@ void launch_grouped_gemm(GemmPlan& plan, int precision, hipStream_t s, const Gem
     launch_shape<double, 128, 64>(plan, c, 3, s, grid_limit, b);
     launch_shape<double, 64, 128>(plan, c, 4, s, grid_limit, b);
     launch_shape<double, 128, 32>(plan, c, 5, s, grid_limit, b);
+    launch_skinny<double>(plan, c, s, grid_limit, b);
   } else {
     launch_shape<float, 64, 64>(plan, c, 0, s, grid_limit, b);
     launch_shape<float, 32, 64>(plan, c, 1, s, grid_limit, b);
@@ -779,6 +957,7 @@ void launch_grouped_gemm(GemmPlan& plan, int precision, hipStream_t s, const Gem
     launch_shape<float, 128, 64>(plan, c, 3, s, grid_limit, b);
     launch_shape<float, 64, 128>(plan, c, 4, s, grid_limit, b);
     launch_shape<float, 128, 32>(plan, c, 5, s, grid_limit, b);
+    launch_skinny<float>(plan, c, s, grid_limit, b);
   }
 }
 

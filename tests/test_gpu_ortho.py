@@ -100,7 +100,13 @@ def test_orthogonalise_matches_oracle(ctx, case):
     got = np.sort(dev.lam(0))[::-1]
     want = np.sort(np.asarray(nl[0]))[::-1]
     assert got.shape == want.shape
-    assert np.allclose(got, want, rtol=1e-8, atol=1e-10)
+    # lambda_j^2 are eigenvalues of the fixed-point product, which device and
+    # oracle both know to about 4e-15 absolute (the transfer eigensolve's
+    # residual and the rounding of its products): |d lambda_j| <= 4e-15 /
+    # (2 lambda_j) on top of the relative 1e-8 -- the small values are
+    # ill-conditioned, and a different summation order in the device's
+    # products moves them within this bound
+    assert np.all(np.abs(got - want) <= 1e-8 * want + 2e-15 / want), (got, want)
     assert abs(dinfo["entropy"] - info["entropy"]) <= 1e-10
     assert np.array_equal(dev.lam(1), np.asarray(nl[1]))
     _check_canonical(dev, rules, real=case != "random")
