@@ -175,7 +175,7 @@ def cpu_baseline(budget_s=20.0, d=64, targets=()):
                 nproc=nproc, cpu_model=_cpu_model(), single_thread=one, all_threads=many,
                 memory_note=("the reference order holds LeftPart (chi d x chi (d+2)/2 entries), RightPart "
                              "(the same) and an intermediate of 1.5 chi^2 d^2 entries at 24 bytes each: "
-                             "about 0.8 TB at chi=2000, d=64"),
+                             "about 1 TB at chi=2000, d=64"),
                 wall_s=round(time.perf_counter() - t_start, 1))
 
 
