@@ -18,8 +18,8 @@ carries the roofline of the dominant kernel (HIP events on the launching
 stream), the chi = 500 section (matvecs, the growth-step eigensolve and
 whole iDMRG growth steps in FP64 and FP32-mixed, the driver's
 orthogonalisation: the largest chi at which a whole iDMRG step fits one
-GPU), the U(1) batching workload (chi = 1024, 21 sectors, d = 32, w = 6)
-and the CPU baseline (oracle/refmatvec, the oracle's C++ reference-order
+GPU), the U(1) batching workload (chi = 1024, 21 sectors, d = 32, w = 6),
+iTEBD quench steps of the d = 64 chain at chi = 1024, and the CPU baseline (oracle/refmatvec, the oracle's C++ reference-order
 matvec, on bounded samples on one host thread and on all of them, with a
 chi-scaling fit).
 
@@ -177,6 +177,44 @@ def cpu_baseline(budget_s=20.0, d=64, targets=()):
                              "(the same) and an intermediate of 1.5 chi^2 d^2 entries at 24 bytes each: "
                              "about 1 TB at chi=2000, d=64"),
                 wall_s=round(time.perf_counter() - t_start, 1))
+
+
+def itebd_section(ctx, cam, workloads, chi, d, timed=2, warm_max=6):
+    """iTEBD real-time quench of the d=64 Z2 coupled chain (J_perp 0 -> 0.1
+    from the product state of level 0, second-order Trotter, dt = 0.01,
+    iTEBD::evolve, TEBD_routines.cpp:120-191) at bond dimension chi: warm-up
+    steps until the bond reaches chi (every value kept up to chi, minS = 0),
+    then `timed` steps, each an odd and an even gate application (gate
+    contraction + the blockwise SVD of two Z2 blocks of (d/2 chi)^2)."""
+    W, rules, phys, bq = workloads.coupled_chain_mpo(ctx, d=d, seed=0, Jperp=0.1)
+    A0, A1, l0, l1 = workloads.product_state_cell(ctx, rules, phys, 0)
+    t = cam.ITEBD(W, A0, A1, l0, l1, 0.01, 2)
+    warm = []
+    for _ in range(warm_max):
+        r = t.evolve(1, chi, 0.0)
+        warm.append({"chi": r["chi"], "seconds": round(r["seconds"], 3)})
+        log(f"itebd warm-up step: chi={r['chi']} {r['seconds']:.2f} s")
+        if r["chi"] >= chi:
+            break
+    steps = []
+    for _ in range(timed):
+        r = t.evolve(1, chi, 0.0)
+        steps.append(r)
+        log(f"itebd step at chi={r['chi']}: {r['seconds']:.2f} s, gate {r['gate_flops'] / 1e12:.1f} TFLOP")
+    t.close()
+    for x in (W, A0, A1):
+        x.close()
+    ctx.release_workspaces()
+    sec = float(np.mean([r["seconds"] for r in steps]))
+    return {"workload": f"iTEBD quench d={d} Z2 coupled chain, J_perp 0 -> 0.1 from level 0, order 2, "
+                        f"dt=0.01, chi={chi}, minS=0",
+            "warmup": warm, "chi": int(steps[-1]["chi"]), "seconds_per_step": round(sec, 3),
+            "gate_tflop_per_step": round(float(np.mean([r["gate_flops"] for r in steps])) / 1e12, 3),
+            "truncation": steps[-1]["truncation"], "entropy": steps[-1]["entropy"],
+            "time": steps[-1]["time"],
+            "note": "per step: odd and even gate contractions on MFMA and the blockwise SVD (leading "
+                    "values of two 32768^2 blocks at chi=1024) in FP64; 200 steps would take "
+                    f"{200 * sec / 60:.0f} min"}
 
 
 def orthogonalise_step(dm, W):
@@ -425,6 +463,9 @@ def main():
     ap.add_argument("--no-eig", action="store_true", help="skip the growth-step eigensolve timing")
     ap.add_argument("--no-idmrg", action="store_true", help="skip the iDMRG step timing")
     ap.add_argument("--no-batching", action="store_true", help="skip the U(1) batching workload")
+    ap.add_argument("--no-itebd", action="store_true", help="skip the iTEBD quench steps at --itebd-chi")
+    ap.add_argument("--itebd-steps", type=int, default=1, help="timed iTEBD steps after the warm-up")
+    ap.add_argument("--itebd-chi", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--idmrg-steps", type=int, default=4,
                     help="growth steps per precision (the bond reaches chi-small at step 2 for d=64)")
@@ -455,6 +496,7 @@ def main():
         if not a.no_idmrg:
             small["idmrg_step"] = idmrg_steps(ctx, cam, workloads, a)
     batching = None if a.no_batching else batching_section(ctx, cam, workloads, a.steps, a.warmup)
+    itebd = None if a.no_itebd else itebd_section(ctx, cam, workloads, a.itebd_chi, a.d, timed=a.itebd_steps)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_budget, a.d, targets=(a.chi_small, a.chi))
     fp64, fp32 = head["fp64"], head["fp32"]
     names = ["L", "W1", "W2", "R"]
@@ -498,6 +540,7 @@ def main():
                               "timed at chi_small below"),
         "chi_small": small,
         "batching": batching,
+        "itebd": itebd,
         "setup_s": head["setup_s"],
     }
     print(json.dumps(line))

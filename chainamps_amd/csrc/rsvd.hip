@@ -14,6 +14,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -198,6 +199,27 @@ bool cholqr_pass(Ctx& ctx, cplx* X, int64_t m, int64_t k, cplx* G, int* dinfo, d
 // passes; GEMM-shaped passes instead of a Householder QR.  Householder QR
 // takes over if a factorization still breaks down (numerically
 // rank-deficient X).
+// X[i] += scale * (a, b), a and b uniform in [-1, 1), scale = rel * the RMS
+// of X's elements (|X|^2 from nrm2[0].x), from a counter-based stream
+__global__ void perturb_kernel(double2* __restrict__ X, int64_t n, uint64_t seed, const double2* __restrict__ nrm2,
+                               double rel) {
+  const double scale = rel * sqrt(fmax(nrm2[0].x, 0.0) / static_cast<double>(n));
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double a = (z >> 11) * (1.0 / 9007199254740992.0);
+    z = z * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z ^= z >> 29;
+    const double b = (z >> 11) * (1.0 / 9007199254740992.0);
+    double2 x = X[i];
+    x.x += scale * (2.0 * a - 1.0);
+    x.y += scale * (2.0 * b - 1.0);
+    X[i] = x;
+  }
+}
+
 void orthonormalise(Ctx& ctx, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& G, DevBuf<int>& dinfo,
                     DevBuf<cplx>& tau) {
   G.ensure(static_cast<size_t>(k) * k);
@@ -213,6 +235,28 @@ void orthonormalise(Ctx& ctx, cplx* X, int64_t m, int64_t k, DevBuf<cplx>& G, De
   if (cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), c) && cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), 0.0) &&
       cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), 0.0) && orthonormal(ctx, X, m, k, G.get()))
     return;
+  // Numerically rank-deficient X (the block itself has rank below k, as in
+  // the first iTEBD steps from a product state): complete it with random
+  // directions of 1e-14 of its element RMS.  The span of its non-negligible
+  // directions moves by about 1e-14 relative (below the acceptance
+  // tolerance), the rest becomes a random orthonormal completion, and the
+  // condition stays within the reach of shifted Cholesky QR3.  Householder
+  // QR (rocSOLVER, panel-by-panel: about 1 s at 34000 x 1536) is the last
+  // resort.
+  {
+    hipStream_t s = ctx.stream;
+    const int64_t n = m * k;
+    DevBuf<cplx> part(static_cast<size_t>(multi_dot_blocks(n)) * 64), nrm(64);
+    launch_multi_dot(X, n, 1, X, n, part.get(), nrm.get(), s);
+    static std::atomic<uint64_t> counter{0};
+    const uint64_t seed = 0x9e3779b97f4a7c15ull * (counter.fetch_add(1) + 1) ^ static_cast<uint64_t>(n);
+    hipLaunchKernelGGL(perturb_kernel, dim3(grid_of(n)), dim3(256), 0, s, reinterpret_cast<double2*>(X), n, seed,
+                       reinterpret_cast<const double2*>(nrm.get()), 1e-14);
+    HIP_CHECK(hipGetLastError());
+    if (cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), c) && cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), 0.0) &&
+        cholqr_pass(ctx, X, m, k, G.get(), dinfo.get(), 0.0) && orthonormal(ctx, X, m, k, G.get()))
+      return;
+  }
   householder_orthonormalise(static_cast<rocblas_handle>(ctx.blas_handle()), X, m, k, tau);
 }
 

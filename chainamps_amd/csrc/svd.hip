@@ -41,9 +41,14 @@ namespace {
 constexpr int64_t kTruncatedMinDim = 1024;
 // Residual bound |A v - s u| of accepted triplets, relative to the block's
 // Frobenius norm: each accepted value is within this of a singular value
-// of the block (the FP64 floor of the residual itself is a few 1e-15 for
-// blocks of 10^4 rows).
+// of the block.  The FP64 floor of the residual itself grows like
+// eps sqrt(m + n) |A| (a few 1e-15 absolute for unit-norm blocks of 10^4
+// rows), so larger blocks get max(5e-14, 2 eps sqrt(m + n)).
 constexpr double kRitzTol = 5e-14;
+double ritz_tol(int64_t m, int64_t n, double weight) {
+  const double floor = 2.0 * std::numeric_limits<double>::epsilon() * std::sqrt(double(m + n));
+  return std::max(kRitzTol, floor) * std::sqrt(weight);
+}
 
 struct FlagJob {
   const double2* in;
@@ -333,18 +338,19 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
       std::vector<double> S;
       int rounds = 0;
       const auto t0 = std::chrono::steady_clock::now();
-      // residuals of the first min(D, kk) triplets are computed; the ones
-      // that must be within the tolerance for acceptance are the first
+      // residuals of the first min(D + 32, kk) triplets are computed; the
+      // ones that must be within the tolerance for acceptance are the first
       // (kept count of the last block of this shape and charge) + margin,
       // or all of them without that hint.  The truncation below verifies
-      // that everything it keeps was within the tolerance.
-      const int64_t ncheck = std::min(D, kk);
+      // that everything it keeps was within the tolerance, with a verified
+      // value below the cut after it.
+      const int64_t ncheck = std::min(D + 32, kk);
       int64_t naccept = ncheck;
       const auto hk = std::make_pair(std::array<int64_t, 3>{g.mg, g.ng, kk}, g.q);
       const auto hit = ctx.svd_kept_hint.find(hk);
       if (hit != ctx.svd_kept_hint.end())
-        naccept = std::min(ncheck, std::max<int64_t>(32, hit->second + std::max<int64_t>(16, hit->second / 8)));
-      const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, ncheck, kRitzTol * std::sqrt(g.weight), 24, S,
+        naccept = std::min(ncheck, std::max<int64_t>(32, hit->second + std::max<int64_t>(32, hit->second / 4)));
+      const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, ncheck, ritz_tol(g.mg, g.ng, g.weight), 24, S,
                                     g.U, g.Vh, &rounds, &g.last_res, naccept, &g.res);
       if (trace)
         std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld (first %lld verified): %s after %d "
@@ -420,34 +426,35 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
     for (size_t gi = 0; gi < glist.size(); ++gi) {
       Group* g = glist[gi];
       if (!g->truncated) continue;
-      bool safe = g->S.back() + g->last_res < below;
-      // a group accepted on a prefix of its triplets: every value it keeps
-      // must be among the verified ones, and each checked value beyond the
-      // verified prefix, raised by its residual, must lie below the cut
+      // Every value a group keeps must be among its verified triplets, and
+      // its last verified value, raised by its residual, must lie below the
+      // cut: the verified Ritz values are the group's leading singular
+      // values, so every later one -- checked but not yet converged, or not
+      // computed at all -- is no larger and stays below the cut too.
       const int64_t nres = static_cast<int64_t>(g->res.size());
-      const double tol = kRitzTol * std::sqrt(g->weight);
-      if (safe && g->nver < nres) {
-        if (kept_of[gi] > g->nver) safe = false;
-        for (int64_t j = g->nver; safe && j < nres; ++j)
-          if (j < kept_of[gi] ? g->res[j] > tol : !(g->S[j] + g->res[j] < below)) safe = false;
-      }
+      const double tol = ritz_tol(g->mg, g->ng, g->weight);
+      bool safe = kept_of[gi] < g->nver;
+      if (safe) safe = g->S[g->nver - 1] + g->res[g->nver - 1] < below;
       if (safe) continue;
       redo = true;
-      if (g->nver < nres) {
-        // verify every checked triplet, as without a kept-count hint
+      // verify every checked triplet (as without a kept-count hint), or as
+      // many as the truncation keeps when that is more (the degeneracy
+      // extension can keep values past D)
+      const int64_t nver = std::min(g->kk, std::max(nres, kept_of[gi] + 16));
+      if (g->nver < nver) {
         std::vector<double> S;
         int rounds = 0;
-        if (truncated_svd(ctx, g->A.get(), g->mg, g->ng, g->kk, nres, tol, 24, S, g->U, g->Vh, &rounds,
-                          &g->last_res, nres, &g->res)) {
+        if (truncated_svd(ctx, g->A.get(), g->mg, g->ng, g->kk, nver, tol, 24, S, g->U, g->Vh, &rounds,
+                          &g->last_res, nver, &g->res)) {
           g->S = std::move(S);
-          g->nver = nres;
+          g->nver = nver;
           double kept = 0.0;
           for (double v : g->S) kept += v * v;
           g->tail = std::max(0.0, g->weight - kept);
           if (trace)
-            std::fprintf(stderr, "svd_truncate: block %lld x %lld re-verified on all %lld checked triplets "
+            std::fprintf(stderr, "svd_truncate: block %lld x %lld re-verified on %lld checked triplets "
                          "(keeps %lld)\n", static_cast<long long>(g->mg), static_cast<long long>(g->ng),
-                         static_cast<long long>(nres), static_cast<long long>(kept_of[gi]));
+                         static_cast<long long>(nver), static_cast<long long>(kept_of[gi]));
           continue;
         }
       }

@@ -73,3 +73,39 @@ def test_idmrg_coupled_chain_mixed(ctx):
         ra, rb = a.grow(24, 1e-12), b.grow(24, 1e-12)
         assert abs(ra["energy"] - rb["energy"]) <= 1e-10 * max(1, abs(ra["energy"])), (step, ra, rb)
     assert rb["matvecs_fp32"] > 0
+
+
+def test_idmrg_growth_setting_mixed_vs_fp64_d64(ctx, capfd):
+    """The production growth step (converge=0: one Krylov cycle of ncv 20
+    at tol 0.1, the predicted start vector) in FP32-mixed precision against
+    FP64, on the d=64 Z2 coupled chain at chi = 64.  From N = 4 on the two
+    QN blocks of theta are 2048 x 2048 with at most 64 values kept, so the
+    leading-values SVD runs inside the driver (checked in its trace).  The
+    FP32 Krylov basis differs from the FP64 one by rounding, so the loose
+    solves agree to their tolerance's second order, not to 1e-10: energies
+    within 1e-7 relative, entropies within 1e-5, the same kept bond."""
+    print("case idmrg growth setting mixed vs fp64, d=64 chi=64", flush=True)
+    import chainamps_amd as cam
+    rules, phys, W, E, sig = models.coupled_chain_model(d=64, seed=0, Jperp=0.1)
+    a = cam.IDMRG(to_lib(ctx, W), converge=False, mixed=False)
+    b = cam.IDMRG(to_lib(ctx, W), converge=False, mixed=True)
+    old = os.environ.get("CAM_SVD_TRACE")
+    os.environ["CAM_SVD_TRACE"] = "1"
+    try:
+        for step in range(4):
+            ra, rb = a.grow(64, 1e-12), b.grow(64, 1e-12)
+            print(f"  N={ra['N']} E {ra['energy']:.12f} / {rb['energy']:.12f} S {ra['entropy']:.9f} / "
+                  f"{rb['entropy']:.9f} chi {ra['chi']}/{rb['chi']} fp32 matvecs {rb['matvecs_fp32']}", flush=True)
+            assert abs(ra["energy"] - rb["energy"]) <= 1e-7 * max(1.0, abs(ra["energy"])), (step, ra, rb)
+            assert abs(ra["entropy"] - rb["entropy"]) <= 1e-5, (step, ra, rb)
+            assert ra["chi"] == rb["chi"]
+    finally:
+        if old is None:
+            del os.environ["CAM_SVD_TRACE"]
+        else:
+            os.environ["CAM_SVD_TRACE"] = old
+    err = capfd.readouterr().err
+    assert "leading" in err, err[-2000:]
+    assert rb["matvecs_fp32"] > 0
+    a.close()
+    b.close()

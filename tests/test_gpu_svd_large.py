@@ -118,3 +118,83 @@ def test_kept_count_hint_reverification(ctx, capfd):
             del os.environ["CAM_SVD_TRACE"]
         else:
             os.environ["CAM_SVD_TRACE"] = old
+
+
+def test_itebd_block_shape_32768(ctx, capfd):
+    """The SVD shape of the d=64 chi=1024 iTEBD quench: two Z2 blocks of
+    32768 x 32768 keeping D = 1024 values (leading-values path, k = 1536).
+    theta = A . B is formed on the device from factors with a prescribed
+    spectrum, A = Q1 diag(s) and B = Q2^H per block (Q1, Q2 random with
+    orthonormal columns, rank 1792 per block, geometric spectra that
+    interleave between the blocks), so the exact values, the discarded
+    weight and the singular vectors' defining relations are known."""
+    import chainamps_amd as cam
+    from chainamps_amd.capi import IndexDesc
+    print("case svd itebd block 2 x 32768^2 keep 1024", flush=True)
+    m, r, D = 32768, 1792, 1024
+    rng = np.random.default_rng(17)
+    j = np.arange(r)
+    spectra = [0.996 ** j, 0.93 * 0.9955 ** j]
+    norm = np.sqrt(sum(np.sum(s ** 2) for s in spectra))
+    spectra = [s / norm for s in spectra]
+
+    def orthonormal_columns(rows, cols):
+        q, _ = np.linalg.qr(rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols)))
+        return q
+    Q1 = [orthonormal_columns(m, r) for _ in range(2)]
+    Q2 = [orthonormal_columns(m, r) for _ in range(2)]
+    print("  factors drawn", flush=True)
+    qrow = (np.arange(2 * m) % 2).reshape(-1, 1)
+    qb = (np.arange(2 * r) % 2).reshape(-1, 1)
+    # A (rows 2m, bond 2r): column c of parity g holds Q1_g[:, c // 2] s_g[c // 2] on the rows of parity g
+    XA = np.empty((2 * r, m), complex)
+    IA = np.empty((2 * r, m), np.int64)
+    XB = np.empty((2 * m, r), complex)
+    IB = np.empty((2 * m, r), np.int64)
+    for g in range(2):
+        XA[g::2] = (Q1[g] * spectra[g]).T
+        IA[g::2] = 2 * np.arange(m) + g
+        XB[g::2] = Q2[g].conj()
+        IB[g::2] = 2 * np.arange(r) + g
+    A = cam.Tensor.from_csc(ctx, [2], [IndexDesc(True, False, qrow), IndexDesc(False, False, qb)], 1,
+                            2 * m, 2 * r, np.arange(2 * r + 1, dtype=np.int64) * m, IA.reshape(-1),
+                            XA.reshape(-1))
+    B = cam.Tensor.from_csc(ctx, [2], [IndexDesc(True, False, qb), IndexDesc(False, False, qrow)], 1,
+                            2 * r, 2 * m, np.arange(2 * m + 1, dtype=np.int64) * r, IB.reshape(-1),
+                            XB.reshape(-1))
+    del XA, IA, XB, IB
+    T = cam.contract(A, False, B, False, [(1, 0)])
+    A.close()
+    B.close()
+    print("  theta formed on the device", flush=True)
+    allv = np.sort(np.concatenate(spectra))[::-1]
+    length, kept, disc = O.truncation_rule(list(allv), D, 0.0)
+    old = os.environ.get("CAM_SVD_TRACE")
+    os.environ["CAM_SVD_TRACE"] = "1"
+    try:
+        U, vals, Vd, d = cam.svd_truncate(T, D, 0.0)
+    finally:
+        if old is None:
+            del os.environ["CAM_SVD_TRACE"]
+        else:
+            os.environ["CAM_SVD_TRACE"] = old
+    T.close()
+    err = capfd.readouterr().err
+    print("  " + "\n  ".join(l for l in err.splitlines() if l.startswith("svd_truncate")), flush=True)
+    assert "leading 1536" in err
+    assert len(vals) == length
+    assert np.max(np.abs(vals - allv[:length])) <= 1e-13
+    assert abs(d - disc) <= 1e-12
+    rules = ChargeRules([2])
+    Um = from_lib(U, rules).matrix.toarray()
+    Vm = from_lib(Vd, rules).matrix.toarray()
+    U.close()
+    Vd.close()
+    assert np.allclose(Um.conj().T @ Um, np.eye(length), atol=1e-12)
+    assert np.allclose(Vm @ Vm.conj().T, np.eye(length), atol=1e-12)
+    # U^H theta V^H = diag(values), block by block from the factors
+    proj = np.zeros((length, length), complex)
+    for g in range(2):
+        rows = slice(g, 2 * m, 2)
+        proj += ((Um[rows].conj().T @ Q1[g]) * spectra[g]) @ (Q2[g].conj().T @ Vm[:, rows].conj().T)
+    assert np.allclose(proj, np.diag(vals), atol=1e-12)
