@@ -186,6 +186,7 @@ def itebd_section(ctx, cam, workloads, chi, d, timed=2, warm_max=6):
     steps until the bond reaches chi (every value kept up to chi, minS = 0),
     then `timed` steps, each an odd and an even gate application (gate
     contraction + the blockwise SVD of two Z2 blocks of (d/2 chi)^2)."""
+    ctx.release_workspaces()       # the eigensolver's Krylov basis of the sections before
     W, rules, phys, bq = workloads.coupled_chain_mpo(ctx, d=d, seed=0, Jperp=0.1)
     A0, A1, l0, l1 = workloads.product_state_cell(ctx, rules, phys, 0)
     t = cam.ITEBD(W, A0, A1, l0, l1, 0.01, 2)
