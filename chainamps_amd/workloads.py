@@ -84,12 +84,12 @@ def coupled_chain_mpo(ctx, d=64, seed=0, Jperp=0.1):
     return tensor_from_dense(ctx, rules, idx, 2, T), rules, phys, bq
 
 
-def product_state_cell(ctx, rules, phys, level=0):
-    """Two-vertex unit cell (A0, A1, lambda0, lambda1) of the product state of
-    basis level `level` (MakeProductStateUnitCell, UnitCell.cpp:190-210):
-    A[s, a_l, a_r] with 1 x 1 bonds; the left bond of A0 carries charge 0,
-    its right bond the level's charge, A1's right bond twice it (Z_N
-    charges wrap)."""
+def product_state_cell_dense(rules, phys, level=0):
+    """Two-vertex unit cell of the product state of basis level `level`
+    (MakeProductStateUnitCell, UnitCell.cpp:190-210): [(index descriptors,
+    dense A[s, a_l, a_r]) for the two vertices] with 1 x 1 bonds; the left
+    bond of A0 carries charge 0, its right bond the level's charge, A1's
+    right bond twice it (Z_N charges wrap)."""
     phys = np.asarray(phys, dtype=np.int64).reshape(-1, len(rules))
     q = phys[level]
     q2 = np.array([(2 * q[i]) % n if n > 0 else 2 * q[i] for i, n in enumerate(rules)])
@@ -99,9 +99,15 @@ def product_state_cell(ctx, rules, phys, level=0):
     def site(ql, qr):
         T = np.zeros((d, 1, 1), complex)
         T[level, 0, 0] = 1.0
-        idx = [IndexDesc(True, True, phys), IndexDesc(True, False, ql), IndexDesc(False, False, qr)]
-        return tensor_from_dense(ctx, rules, idx, 2, T)
-    return site(z, q.reshape(1, -1)), site(q.reshape(1, -1), q2.reshape(1, -1)), np.ones(1), np.ones(1)
+        return [IndexDesc(True, True, phys), IndexDesc(True, False, ql), IndexDesc(False, False, qr)], T
+    return [site(z, q.reshape(1, -1)), site(q.reshape(1, -1), q2.reshape(1, -1))]
+
+
+def product_state_cell(ctx, rules, phys, level=0):
+    """(A0, A1, lambda0, lambda1) of product_state_cell_dense on the device."""
+    (i0, T0), (i1, T1) = product_state_cell_dense(rules, phys, level)
+    return (tensor_from_dense(ctx, rules, i0, 2, T0), tensor_from_dense(ctx, rules, i1, 2, T1),
+            np.ones(1), np.ones(1))
 
 
 def coupled_chain_generators(csc, d=64, seed=0):

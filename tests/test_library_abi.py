@@ -98,3 +98,21 @@ def test_coupled_chain_generators():
         workloads.coupled_chain_generators(
             (W00.shape[0], W00.shape[1], W00.indptr.astype(np.int64), W00.indices.astype(np.int64), W00.data),
             d, seed=seed)
+
+
+@pytest.mark.parametrize("rules,level", [([2], 0), ([2], 1), ([0], 3)])
+def test_product_state_cell_matches_oracle(rules, level):
+    """chainamps_amd.workloads (the bench's iTEBD start) builds the oracle's
+    product-state unit cell: charges, directions and entries."""
+    from chainamps_amd import workloads
+    from oracle import tebd
+    from oracle.qn import ChargeRules
+    d = 6
+    phys = (np.arange(d) % 2).reshape(-1, 1) if rules[0] == 2 else (np.arange(d) - 2).reshape(-1, 1)
+    got = workloads.product_state_cell_dense(rules, phys, level)
+    want = tebd.product_state_cell(ChargeRules(rules), phys, level)[:2]
+    for (idx, T), W in zip(got, want):
+        assert np.array_equal(T, W.to_dense_tensor())
+        for a, b in zip(idx, W.indices):
+            assert bool(a.inward) == bool(b.inward) and bool(a.physical) == bool(b.physical)
+            assert np.array_equal(np.asarray(a.charges).reshape(-1, 1), np.asarray(b.charges).reshape(-1, 1))
