@@ -37,9 +37,17 @@ struct Ctx {
   // the next block of that shape; more are checked when the truncation
   // keeps more)
   std::map<std::pair<std::array<int64_t, 3>, Charge>, int64_t> svd_kept_hint;
+  // A second stream and rocBLAS handle on the same device (created on first
+  // use, default priority) for work one call runs on two lanes at once: the
+  // leading-values SVD of two large blocks.
+  std::unique_ptr<Ctx> aux;
   explicit Ctx(int dev);
   ~Ctx();
   void* blas_handle();
+  Ctx& aux_lane() {
+    if (!aux) aux.reset(new Ctx(device));
+    return *aux;
+  }
 };
 
 bool indices_match(const QIndex& a, bool conjA, const QIndex& b, bool conjB);

@@ -29,6 +29,8 @@
 #include <map>
 #include <cstdlib>
 #include <numeric>
+#include <exception>
+#include <thread>
 
 #include "solvers.hpp"
 
@@ -45,6 +47,16 @@ constexpr int64_t kTruncatedMinDim = 1024;
 // eps sqrt(m + n) |A| (a few 1e-15 absolute for unit-norm blocks of 10^4
 // rows), so larger blocks get max(5e-14, 2 eps sqrt(m + n)).
 constexpr double kRitzTol = 5e-14;
+// CAM_SVD_SERIAL=1 decomposes the large blocks one after the other on the
+// context's stream (no second lane)
+bool serial_lanes() {
+  static const bool serial = [] {
+    const char* e = std::getenv("CAM_SVD_SERIAL");
+    return e && *e && *e != '0';
+  }();
+  return serial;
+}
+
 double ritz_tol(int64_t m, int64_t n, double weight) {
   const double floor = 2.0 * std::numeric_limits<double>::epsilon() * std::sqrt(double(m + n));
   return std::max(kRitzTol, floor) * std::sqrt(weight);
@@ -277,6 +289,13 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
     CAM_REQUIRE(info == 0, "blockwise SVD did not converge");
     g.A.reset();
   };
+  struct Pending {
+    Group* g;
+    int64_t kk, ncheck, naccept;
+    double tol;
+    bool ok;
+  };
+  std::vector<Pending> pending;
   for (size_t gi = 0; gi < glist.size(); ++gi) {
     Group& g = *glist[gi];
     if (g.mg == 0 || g.ng == 0) continue;
@@ -335,9 +354,6 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
     const int64_t kfull = std::min(g.mg, g.ng);
     const int64_t kk = D > 0 ? std::min(kfull, D + std::max<int64_t>(D / 2, 32)) : kfull;
     if (!exact_only && D > 0 && kfull >= kTruncatedMinDim && 2 * kk <= kfull) {
-      std::vector<double> S;
-      int rounds = 0;
-      const auto t0 = std::chrono::steady_clock::now();
       // residuals of the first min(D + 32, kk) triplets are computed; the
       // ones that must be within the tolerance for acceptance are the first
       // (kept count of the last block of this shape and charge) + margin,
@@ -350,26 +366,78 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
       const auto hit = ctx.svd_kept_hint.find(hk);
       if (hit != ctx.svd_kept_hint.end())
         naccept = std::min(ncheck, std::max<int64_t>(32, hit->second + std::max<int64_t>(32, hit->second / 4)));
-      const bool ok = truncated_svd(ctx, g.A.get(), g.mg, g.ng, kk, ncheck, ritz_tol(g.mg, g.ng, g.weight), 24, S,
-                                    g.U, g.Vh, &rounds, &g.last_res, naccept, &g.res);
-      if (trace)
-        std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld (first %lld verified): %s after %d "
-                     "rounds, %.3f s\n",
-                     static_cast<long long>(g.mg), static_cast<long long>(g.ng), static_cast<long long>(kk),
-                     static_cast<long long>(naccept), ok ? "accepted" : "not accepted", rounds,
-                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
-      if (ok) {
-        g.S = std::move(S);
-        g.k = kk;
-        g.kk = kk;
-        g.nver = naccept;
-        g.truncated = true;
-        double kept = 0.0;
-        for (double v : g.S) kept += v * v;
-        g.tail = std::max(0.0, g.weight - kept);
-        continue;   // g.A is kept in case the full decomposition is needed
-      }
+      pending.push_back(Pending{&g, kk, ncheck, naccept, ritz_tol(g.mg, g.ng, g.weight), false});
+      continue;   // decomposed below, with the other large blocks
     }
+    const auto t1 = std::chrono::steady_clock::now();
+    full_svd(g);
+    if (trace) {
+      HIP_CHECK(hipStreamSynchronize(s));
+      std::fprintf(stderr, "svd_truncate: block %lld x %lld, full decomposition, %.3f s\n",
+                   static_cast<long long>(g.mg), static_cast<long long>(g.ng),
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
+    }
+  }
+
+  // ---- leading values of the large blocks ----------------------------------
+  // Two lanes: every other block runs on the context's auxiliary stream and
+  // rocBLAS handle from a second host thread, so the latency-bound parts of
+  // one block (Cholesky factorisations, the Jordan-Wielandt core, the host
+  // synchronisations of the residual checks) overlap the other block's
+  // GEMMs.  Both streams have the default priority.  The group matrices were
+  // completed on the main stream, which is synchronised above.
+  auto run_pending = [&](Ctx& lane, Pending& pj) {
+    Group& g = *pj.g;
+    std::vector<double> S;
+    int rounds = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    pj.ok = truncated_svd(lane, g.A.get(), g.mg, g.ng, pj.kk, pj.ncheck, pj.tol, 24, S, g.U, g.Vh, &rounds,
+                          &g.last_res, pj.naccept, &g.res);
+    HIP_CHECK(hipStreamSynchronize(lane.stream));
+    if (trace)
+      std::fprintf(stderr, "svd_truncate: block %lld x %lld, leading %lld (first %lld verified): %s after %d "
+                   "rounds, %.3f s\n",
+                   static_cast<long long>(g.mg), static_cast<long long>(g.ng), static_cast<long long>(pj.kk),
+                   static_cast<long long>(pj.naccept), pj.ok ? "accepted" : "not accepted", rounds,
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    if (pj.ok) {
+      g.S = std::move(S);
+      g.k = pj.kk;
+      g.kk = pj.kk;
+      g.nver = pj.naccept;
+      g.truncated = true;
+      double kept = 0.0;
+      for (double v : g.S) kept += v * v;
+      g.tail = std::max(0.0, g.weight - kept);
+    }
+  };
+  if (pending.size() >= 2 && !serial_lanes()) {
+    Ctx& aux = ctx.aux_lane();
+    std::exception_ptr err;
+    std::thread helper([&]() {
+      try {
+        HIP_CHECK(hipSetDevice(aux.device));
+        for (size_t i = 1; i < pending.size(); i += 2) run_pending(aux, pending[i]);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    std::exception_ptr mine;
+    try {
+      for (size_t i = 0; i < pending.size(); i += 2) run_pending(ctx, pending[i]);
+    } catch (...) {
+      mine = std::current_exception();
+    }
+    helper.join();
+    if (mine) std::rethrow_exception(mine);
+    if (err) std::rethrow_exception(err);
+  } else {
+    for (Pending& pj : pending) run_pending(ctx, pj);
+  }
+  for (Pending& pj : pending) {
+    if (pj.ok) continue;   // g.A is kept in case the full decomposition is needed
+    Group& g = *pj.g;
+    g.kk = pj.kk;
     const auto t1 = std::chrono::steady_clock::now();
     full_svd(g);
     if (trace) {
