@@ -11,9 +11,11 @@
 // pair is the same).
 //
 // All vectors stay on the device in the block layout; Krylov vectors are
-// kept fully orthogonal with two passes of classical Gram-Schmidt in FP64;
-// the tridiagonal eigenproblem is solved in FP64 on the device with
-// rocSOLVER dsteqr.
+// kept fully orthogonal: each new vector is first projected off the last
+// two basis vectors (the large alpha and beta components of the three-term
+// recurrence), then off the whole basis in one classical Gram-Schmidt pass
+// (FP64 sums); the tridiagonal eigenproblem is solved in FP64 on the device
+// with rocSOLVER dsteqr.
 //
 // Mixed precision: each restart starts from the FP64 Ritz vector x, whose
 // FP64 matvec gives the exact Rayleigh quotient and residual r; the Krylov
@@ -53,7 +55,7 @@ namespace {
 
 // Lanczos step tail on the device: beta = |w| from the norm slot; the step
 // breaks down when beta <= 16 eps max(1, |alpha|), alpha = Re of the summed
-// CGS2 coefficients c1[j] + c2[j].  Writes v_{j+1} = w / beta (zero after a
+// projection coefficients on v_j, c1[j] (local pass) + c2[j] (full pass).  Writes v_{j+1} = w / beta (zero after a
 // breakdown) and the decision {beta, flag} for the host.
 template <typename T>
 __global__ void lanczos_next_kernel(const T* __restrict__ w, T* __restrict__ next, int64_t n,
@@ -85,7 +87,7 @@ struct Lanczos {
   float2* V32 = nullptr;
   float2* w32 = nullptr;
   DevBuf<cplx> coef, partial;
-  DevBuf<cplx> C1, C2, Nrm, Dec;   // per-step CGS2 coefficients, norms, decisions
+  DevBuf<cplx> C1, C2, Nrm, Dec;   // per-step projection coefficients, norms, decisions
   DevBuf<double> dD, dE, dC;
   DevBuf<int> dinfo;
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
@@ -124,8 +126,18 @@ struct Lanczos {
 
   // Steps j = 1 .. m-1 of one Krylov cycle with every scalar kept on the
   // device (col(1) must hold the normalised residual of step 0): matvec,
-  // two CGS passes, norm, normalisation.  No host synchronisation, so the
-  // whole sequence is one hipGraph.
+  // projection off v_{j-1}, v_j, one full Gram-Schmidt pass, norm,
+  // normalisation.  No host synchronisation, so the whole sequence is one
+  // hipGraph.
+  //
+  // H v_j lies in span(v_{j-1}, v_j, v_{j+1}) up to rounding (and, in an
+  // FP32 cycle, the FP32 error of the matvec), so the local pass removes
+  // everything large; what is left along the older vectors is small next to
+  // |w| unless the space is nearly invariant (which the breakdown test
+  // catches), so the single full pass suffers no cancellation and restores
+  // orthogonality to working precision -- where a full pass applied to
+  // H v_j directly cancels |alpha| / beta of the vector and needs a second
+  // pass.  Per step this streams 2j + 15 vectors instead of 4j + 13.
   void enqueue_cycle(int prec) {
     const double eps16 = 16 * std::numeric_limits<double>::epsilon();
     if (prec == 1) {
@@ -135,8 +147,8 @@ struct Lanczos {
         H.apply_blocked32(col32(j), w32);
         cplx* c1 = C1.get() + j * 64;
         cplx* c2 = C2.get() + j * 64;
-        launch_multi_dot32(V32, n, j + 1, w32, n, partial.get(), c1, s);
-        launch_multi_axpy32(V32, n, j + 1, c1, w32, n, s);
+        launch_multi_dot32(col32(j - 1), n, 2, w32, n, partial.get(), c1 + (j - 1), s);
+        launch_multi_axpy32(col32(j - 1), n, 2, c1 + (j - 1), w32, n, s);
         launch_multi_dot32(V32, n, j + 1, w32, n, partial.get(), c2, s);
         launch_multi_axpy32(V32, n, j + 1, c2, w32, n, s);
         launch_multi_dot32(w32, n, 1, w32, n, partial.get(), Nrm.get() + j, s);
@@ -153,11 +165,10 @@ struct Lanczos {
       H.apply(col(j), w.get(), prec);
       cplx* c1 = C1.get() + j * 64;
       cplx* c2 = C2.get() + j * 64;
-      // CGS2: two passes of c = V^H w, w -= V c, then |w|^2.  (A fused
-      // update-and-dot sweep reads the basis once less but, holding a row of
-      // V per thread, ran at half the bandwidth of the two separate sweeps.)
-      launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c1, s);
-      launch_multi_axpy(V.get(), n, j + 1, c1, w.get(), n, s);
+      // local pass c = [v_{j-1} v_j]^H w, w -= [v_{j-1} v_j] c; full pass
+      // c = V^H w, w -= V c; then |w|^2
+      launch_multi_dot(col(j - 1), n, 2, w.get(), n, partial.get(), c1 + (j - 1), s);
+      launch_multi_axpy(col(j - 1), n, 2, c1 + (j - 1), w.get(), n, s);
       launch_multi_dot(V.get(), n, j + 1, w.get(), n, partial.get(), c2, s);
       launch_multi_axpy(V.get(), n, j + 1, c2, w.get(), n, s);
       launch_multi_dot(w.get(), n, 1, w.get(), n, partial.get(), Nrm.get() + j, s);
