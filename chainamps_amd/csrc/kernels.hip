@@ -197,12 +197,21 @@ __global__ void multi_axpy_kernel(const TV* __restrict__ V, int64_t ldv, int k,
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     double r = 0, im = 0;
-#pragma unroll 4
-    for (int j = 0; j < k; ++j) {
-      const double2 a = ld2(V + i + j * ldv);
-      const double2 b = cc[j];
-      r += a.x * b.x - a.y * b.y;
-      im += a.x * b.y + a.y * b.x;
+    // columns in chunks of eight: the eight loads of a chunk are issued
+    // before any of them is used (the guard is uniform across the wave)
+    for (int j0 = 0; j0 < k; j0 += 8) {
+      double2 a[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (j0 + q < k) a[q] = ld2(V + i + (j0 + q) * ldv);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (j0 + q < k) {
+          const double2 b = cc[j0 + q];
+          r += a[q].x * b.x - a[q].y * b.y;
+          im += a[q].x * b.y + a[q].y * b.x;
+        }
+      }
     }
     if (mode) {
       double2 o = ld2(w + i);

@@ -22,6 +22,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -41,14 +42,17 @@ namespace {
 // Blocks with min(rows, cols) >= this and room for at most half their values
 // use the leading-values decomposition (CAM_SVD_EXACT=1 disables it).
 constexpr int64_t kTruncatedMinDim = 1024;
+// a call uses the second lane when at least two of its decompositions have
+// this many elements (calls on tiny blocks stay on one lane and one thread)
+constexpr int64_t kLaneMinElems = int64_t(1) << 14;
 // Residual bound |A v - s u| of accepted triplets, relative to the block's
 // Frobenius norm: each accepted value is within this of a singular value
 // of the block.  The FP64 floor of the residual itself grows like
 // eps sqrt(m + n) |A| (a few 1e-15 absolute for unit-norm blocks of 10^4
 // rows), so larger blocks get max(5e-14, 2 eps sqrt(m + n)).
 constexpr double kRitzTol = 5e-14;
-// CAM_SVD_SERIAL=1 decomposes the large blocks one after the other on the
-// context's stream (no second lane)
+// CAM_SVD_SERIAL=1 runs every decomposition of a call one after the other
+// on the context's stream (no second lane)
 bool serial_lanes() {
   static const bool serial = [] {
     const char* e = std::getenv("CAM_SVD_SERIAL");
@@ -253,47 +257,64 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
   }
 
   // ---- assemble dense group matrices and decompose -----------------------
-  rocblas_handle h = static_cast<rocblas_handle>(ctx.blas_handle());
-  DevBuf<int> dinfo(1);
-  DevBuf<double> dE, dS;
   DevBuf<cplx> partial(static_cast<size_t>(multi_dot_blocks(int64_t(1) << 40))), dcoef(64);   // k = 1 below
   const char* ev = std::getenv("CAM_SVD_EXACT");
   const bool exact_only = ev && *ev && *ev != '0';
   // CAM_SVD_TRACE=1 prints the path and time of every decomposed block
   const char* tv = std::getenv("CAM_SVD_TRACE");
   const bool trace = tv && *tv && *tv != '0';
+  // per-lane result buffers of the full decompositions (grown before the
+  // lanes start, so that no lane frees device memory while the other runs)
+  struct LaneBufs {
+    DevBuf<double> dS, dE;
+    DevBuf<int> dinfo;
+  };
+  LaneBufs lane_bufs[2];
+  for (LaneBufs& lb : lane_bufs) lb.dinfo.alloc(1);
   // the full (economy) decomposition of one group, as the reference's
-  // per-block zgesdd
-  auto full_svd = [&](Group& g) {
+  // per-block zgesdd, on the stream and rocBLAS handle of `lane`
+  auto full_svd = [&](Ctx& lane, LaneBufs& lb, Group& g) {
+    const hipStream_t ls = lane.stream;
     g.k = std::min(g.mg, g.ng);
     g.truncated = false;
     g.tail = 0.0;
     // values only: no singular vectors (U, V^dagger are 1-element placeholders)
     g.U.alloc(values_only ? 1 : g.mg * g.k);
     g.Vh.alloc(values_only ? 1 : g.k * g.ng);
-    dS.ensure(g.k);
-    dE.ensure(std::max<int64_t>(g.k, 1));
+    lb.dS.ensure(g.k);
+    lb.dE.ensure(std::max<int64_t>(g.k, 1));
     const rocblas_svect sv = values_only ? rocblas_svect_none : rocblas_svect_singular;
     const rocblas_status st = rocsolver_zgesvd(
-        h, sv, sv, static_cast<rocblas_int>(g.mg),
+        static_cast<rocblas_handle>(lane.blas_handle()), sv, sv, static_cast<rocblas_int>(g.mg),
         static_cast<rocblas_int>(g.ng), reinterpret_cast<rocblas_double_complex*>(g.A.get()),
-        static_cast<rocblas_int>(g.mg), dS.get(), reinterpret_cast<rocblas_double_complex*>(g.U.get()),
+        static_cast<rocblas_int>(g.mg), lb.dS.get(), reinterpret_cast<rocblas_double_complex*>(g.U.get()),
         static_cast<rocblas_int>(values_only ? 1 : g.mg), reinterpret_cast<rocblas_double_complex*>(g.Vh.get()),
-        static_cast<rocblas_int>(values_only ? 1 : g.k), dE.get(), rocblas_outofplace, dinfo.get());
+        static_cast<rocblas_int>(values_only ? 1 : g.k), lb.dE.get(), rocblas_outofplace, lb.dinfo.get());
     CAM_REQUIRE(st == rocblas_status_success, "rocsolver_zgesvd failed");
     g.S.resize(g.k);
     int info = 0;
-    HIP_CHECK(hipMemcpyAsync(g.S.data(), dS.get(), g.k * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipMemcpyAsync(g.S.data(), lb.dS.get(), g.k * sizeof(double), hipMemcpyDeviceToHost, ls));
+    HIP_CHECK(hipMemcpyAsync(&info, lb.dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, ls));
+    HIP_CHECK(hipStreamSynchronize(ls));
     CAM_REQUIRE(info == 0, "blockwise SVD did not converge");
     g.A.reset();
   };
+  auto timed_full_svd = [&](Ctx& lane, LaneBufs& lb, Group& g) {
+    const auto t1 = std::chrono::steady_clock::now();
+    full_svd(lane, lb, g);
+    if (trace)
+      std::fprintf(stderr, "svd_truncate: block %lld x %lld, full decomposition, %.3f s\n",
+                   static_cast<long long>(g.mg), static_cast<long long>(g.ng),
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
+  };
+  // one decomposition to run: the leading values of a large block, or a
+  // full decomposition
   struct Pending {
     Group* g;
     int64_t kk, ncheck, naccept;
     double tol;
     bool ok;
+    bool full;
   };
   std::vector<Pending> pending;
   for (size_t gi = 0; gi < glist.size(); ++gi) {
@@ -366,27 +387,22 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
       const auto hit = ctx.svd_kept_hint.find(hk);
       if (hit != ctx.svd_kept_hint.end())
         naccept = std::min(ncheck, std::max<int64_t>(32, hit->second + std::max<int64_t>(32, hit->second / 4)));
-      pending.push_back(Pending{&g, kk, ncheck, naccept, ritz_tol(g.mg, g.ng, g.weight), false});
-      continue;   // decomposed below, with the other large blocks
+      pending.push_back(Pending{&g, kk, ncheck, naccept, ritz_tol(g.mg, g.ng, g.weight), false, false});
+      continue;
     }
-    const auto t1 = std::chrono::steady_clock::now();
-    full_svd(g);
-    if (trace) {
-      HIP_CHECK(hipStreamSynchronize(s));
-      std::fprintf(stderr, "svd_truncate: block %lld x %lld, full decomposition, %.3f s\n",
-                   static_cast<long long>(g.mg), static_cast<long long>(g.ng),
-                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
-    }
+    pending.push_back(Pending{&g, 0, 0, 0, 0.0, false, true});
   }
 
-  // ---- leading values of the large blocks ----------------------------------
-  // Two lanes: every other block runs on the context's auxiliary stream and
-  // rocBLAS handle from a second host thread, so the latency-bound parts of
-  // one block (Cholesky factorisations, the Jordan-Wielandt core, the host
-  // synchronisations of the residual checks) overlap the other block's
-  // GEMMs.  Both streams have the default priority.  The group matrices were
-  // completed on the main stream, which is synchronised above.
-  auto run_pending = [&](Ctx& lane, Pending& pj) {
+  // ---- decompositions --------------------------------------------------------
+  // Two lanes: the context's stream and rocBLAS handle, and an auxiliary
+  // pair driven from a second host thread, pull the decompositions from one
+  // queue, largest first.  The latency-bound parts of one (Cholesky
+  // factorisations, the Jordan-Wielandt core, rocSOLVER's bidiagonal sweeps,
+  // the host synchronisations of the residual checks) then overlap the other
+  // one's GEMMs.  Both streams have the default priority.  Every group matrix
+  // was completed on the main stream, which is synchronised above; each
+  // decomposition touches its own group only.
+  auto run_truncated = [&](Ctx& lane, Pending& pj) {
     Group& g = *pj.g;
     std::vector<double> S;
     int rounds = 0;
@@ -411,41 +427,60 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
       g.tail = std::max(0.0, g.weight - kept);
     }
   };
-  if (pending.size() >= 2 && !serial_lanes()) {
+  auto run_item = [&](Ctx& lane, LaneBufs& lb, Pending& pj) {
+    if (pj.full) timed_full_svd(lane, lb, *pj.g);
+    else run_truncated(lane, pj);
+  };
+  // cost order: the leading values of a block take about 8 k m n (four
+  // rounds of two products), a full decomposition about min(m, n) m n
+  auto cost = [](const Pending& pj) {
+    const double mn = double(pj.g->mg) * double(pj.g->ng);
+    return pj.full ? mn * double(std::min(pj.g->mg, pj.g->ng)) : 8.0 * mn * double(pj.kk);
+  };
+  std::stable_sort(pending.begin(), pending.end(),
+                   [&](const Pending& a, const Pending& b) { return cost(a) > cost(b); });
+  int64_t kmax = 1, nbig = 0;
+  for (const Pending& pj : pending) {
+    if (pj.full) kmax = std::max(kmax, std::min(pj.g->mg, pj.g->ng));
+    if (pj.g->mg * pj.g->ng >= kLaneMinElems) ++nbig;
+  }
+  for (LaneBufs& lb : lane_bufs) {
+    lb.dS.ensure(kmax);
+    lb.dE.ensure(kmax);
+  }
+  if (nbig >= 2 && !serial_lanes()) {
     Ctx& aux = ctx.aux_lane();
+    std::atomic<size_t> next{0};
+    auto drain = [&](Ctx& lane, LaneBufs& lb) {
+      for (size_t i = next++; i < pending.size(); i = next++) run_item(lane, lb, pending[i]);
+    };
     std::exception_ptr err;
     std::thread helper([&]() {
       try {
         HIP_CHECK(hipSetDevice(aux.device));
-        for (size_t i = 1; i < pending.size(); i += 2) run_pending(aux, pending[i]);
+        drain(aux, lane_bufs[1]);
       } catch (...) {
+        next = pending.size();   // the other lane stops after its current item
         err = std::current_exception();
       }
     });
     std::exception_ptr mine;
     try {
-      for (size_t i = 0; i < pending.size(); i += 2) run_pending(ctx, pending[i]);
+      drain(ctx, lane_bufs[0]);
     } catch (...) {
+      next = pending.size();
       mine = std::current_exception();
     }
     helper.join();
     if (mine) std::rethrow_exception(mine);
     if (err) std::rethrow_exception(err);
   } else {
-    for (Pending& pj : pending) run_pending(ctx, pj);
+    for (Pending& pj : pending) run_item(ctx, lane_bufs[0], pj);
   }
   for (Pending& pj : pending) {
-    if (pj.ok) continue;   // g.A is kept in case the full decomposition is needed
-    Group& g = *pj.g;
-    g.kk = pj.kk;
-    const auto t1 = std::chrono::steady_clock::now();
-    full_svd(g);
-    if (trace) {
-      HIP_CHECK(hipStreamSynchronize(s));
-      std::fprintf(stderr, "svd_truncate: block %lld x %lld, full decomposition, %.3f s\n",
-                   static_cast<long long>(g.mg), static_cast<long long>(g.ng),
-                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
-    }
+    if (pj.full || pj.ok) continue;   // g.A is kept in case the full decomposition is needed
+    pj.g->kk = pj.kk;
+    timed_full_svd(ctx, lane_bufs[0], *pj.g);
   }
 
   // ---- global sort and truncation rule --------------------------------------
@@ -526,7 +561,7 @@ void svd_core(Ctx& ctx, const BSTensor& th, int64_t D, double max_trunc, TensorP
           continue;
         }
       }
-      full_svd(*g);   // each pass turns at least one group exact: at most 2 #groups passes
+      full_svd(ctx, lane_bufs[0], *g);   // each pass turns at least one group exact: at most 2 #groups passes
     }
     if (!redo) break;
   }
