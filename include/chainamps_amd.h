@@ -15,7 +15,10 @@
  * columns (MPX_matrix.hpp:61-66, sparse_interface.cpp:85-91).
  *
  * Threading: one context per host thread; a context is not re-entrant.
- * Calls taking host buffers synchronise before returning.
+ * Calls taking host buffers synchronise before returning.  A blockwise SVD
+ * with two or more large blocks runs part of them on a helper thread with a
+ * second stream and rocBLAS handle of the same context (default priority),
+ * joined before the call returns; CAM_SVD_SERIAL=1 keeps one stream.
  */
 #ifndef CHAINAMPS_AMD_H
 #define CHAINAMPS_AMD_H
