@@ -26,8 +26,8 @@ pytestmark = pytest.mark.gpu
 HULTHEN = 0.25 - np.log(2.0)
 
 
-def test_cfg1_heisenberg_chi100_20_iterations(ctx):
-    print("case cfg1 heisenberg chi=100 x 20", flush=True)
+def test_heisenberg_chi100_20_iterations(ctx):
+    print("case heisenberg chi=100 x 20", flush=True)
     import chainamps_amd as cam
     rules, phys, W = models.heisenberg_mpo()
     ref = dmrg.IDMRG(W, converge=True)
@@ -70,8 +70,8 @@ def _batching_oracle(chi):
             for idx, T, nr in parts]
 
 
-def test_cfg2_batching_vs_oracle(ctx):
-    print("case cfg2 batching chi=96 vs oracle", flush=True)
+def test_u1_batching_chi96_vs_oracle(ctx):
+    print("case u1 batching chi=96 vs oracle", flush=True)
     import chainamps_amd as cam
     from chainamps_amd import workloads
     LB, W, RB = _batching_oracle(96)
@@ -88,9 +88,9 @@ def test_cfg2_batching_vs_oracle(ctx):
     assert rel_err(H.apply(v), ref) <= 1e-12
 
 
-def test_cfg2_batching_full_size(ctx):
+def test_u1_batching_chi1024_full_size(ctx):
     """chi = 1024 (the workload the benchmark's batching entry times)."""
-    print("case cfg2 batching chi=1024 full size", flush=True)
+    print("case u1 batching chi=1024 full size", flush=True)
     import torch
     import chainamps_amd as cam
     from chainamps_amd import workloads
