@@ -137,7 +137,9 @@ struct Lanczos {
   // catches), so the single full pass suffers no cancellation and restores
   // orthogonality to working precision -- where a full pass applied to
   // H v_j directly cancels |alpha| / beta of the vector and needs a second
-  // pass.  Per step this streams 2j + 15 vectors instead of 4j + 13.
+  // pass.  Per step this streams 2j + 15 + ceil((j+1)/8) vectors (the dot
+  // sweeps re-read w once per eight basis vectors) instead of the
+  // 4j + 12 + 2 ceil((j+1)/8) of two full passes.
   void enqueue_cycle(int prec) {
     const double eps16 = 16 * std::numeric_limits<double>::epsilon();
     if (prec == 1) {
